@@ -1,0 +1,633 @@
+// HIP kernels of the HOW-MHD time step for gfx950.
+//
+// Device layout: only active cells are stored, one array per variable,
+// x fastest: index (z*ny + y)*nx + x.  A neighbour outside the active range is
+// read through the boundary map (periodic wrap or open clamp), which yields
+// exactly the ghost values src/bound.cpp and the serial fills of
+// src/fluxct.cpp would have written (axes with n >= 3; a degenerate n = 1 axis
+// is never stencilled by the time step).
+//
+// Per SSPRK stage iw the kernels are:
+//   k_sweep<X|Y|Z>  fused primit + eigensystem + characteristic LLF-WENO5:
+//                   interface fluxes dF (7 per interface) and CT flux terms
+//                   (src/ssprk.cpp:458-997, 1000-1272)
+//   k_emf2, k_emf   corner EMF build and smoothing (src/fluxct.cpp:809-840)
+//   k_faces         face-B RK combination + 4th-order staggered curl
+//                   (src/fluxct.cpp:848-877, 111-135)
+//   k_combine       stage combination k1*q_init + k3*q_prev + dX + dY + dZ,
+//                   face->centre B (src/fluxct.cpp:882-897) and the floors of
+//                   prot (src/prot.cpp:10-51)
+// and per step k_cfl (src/tstep.cpp:28-83).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "mhd_math.hpp"
+
+namespace howmhd {
+
+// ------------------------------------------------------------------ geometry
+struct Geom {
+    int n[3];      // active cells per axis
+    int per[3];    // 1 periodic, 0 open
+    int sx, sy, sz;
+    long long ncell;
+};
+
+// Active index a in [-3, n+2] -> stored index, following src/bound.cpp.
+__device__ __forceinline__ int bmap(int a, int n, int per) {
+    if (n == 1) return 0;
+    if (per) return a < 0 ? a + n : (a >= n ? a - n : a);
+    return a < 0 ? 0 : (a >= n ? n - 1 : a);
+}
+
+// Sweep permutation of src/ssprk.cpp:381-403, 1035-1065, 1126-1156: sweep slot m
+// holds q variable PERM[dir][m]; order (rho, Mn, Mt1, Mt2, Bn, Bt1, Bt2, E).
+__device__ __constant__ const int kPerm[3][8] = {
+    {0, 1, 2, 3, 4, 5, 6, 7},
+    {0, 2, 3, 1, 5, 6, 4, 7},
+    {0, 3, 1, 2, 6, 4, 5, 7},
+};
+
+struct Phys {
+    double gam, rhomin, pgmin;
+};
+
+// ------------------------------------------------------------- sweep tiling
+// X sweep: 64 interfaces along x per row, 4 rows (y) per block.
+// Y/Z sweep: 32 x-columns, 8 interfaces along the sweep axis per block.
+template <int DIR> struct SweepTile;
+template <> struct SweepTile<0> {
+    static constexpr int LANES = 64, ROWS = 4, CELLS = (64 + 5) * 4;
+};
+template <> struct SweepTile<1> {
+    static constexpr int LANES = 32, ROWS = 8, CELLS = 32 * (8 + 5);
+};
+template <> struct SweepTile<2> {
+    static constexpr int LANES = 32, ROWS = 8, CELLS = 32 * (8 + 5);
+};
+
+template <typename R>
+struct SweepArgs {
+    const double* q;    // source stage, 8 arrays of ncell
+    R* dF;              // 7 x n_if planes (see dF_index)
+    R* ct_a;            // CT term "bsy": fsy (x), gsz (y), hsx (z)
+    R* ct_b;            // CT term "bsz": fsz (x), gsx (y), hsy (z)
+    Geom g;
+    Phys ph;
+};
+
+// dF layout per direction: component-major, interface index t = 0..n along the
+// sweep axis (t is the left face of active cell t):
+//   x: ((c*nz + z)*ny + y)*(nx+1) + t
+//   y: ((c*nz + z)*(ny+1) + t)*nx + x
+//   z: ((c*(nz+1) + t)*ny + y)*nx + x
+template <int DIR>
+__device__ __forceinline__ long long dF_index(const Geom& g, int c, int t, int x, int y, int z) {
+    if constexpr (DIR == 0) return (((long long)c * g.n[2] + z) * g.n[1] + y) * (g.n[0] + 1) + t;
+    else if constexpr (DIR == 1) return (((long long)c * g.n[2] + z) * (g.n[1] + 1) + t) * g.n[0] + x;
+    else return (((long long)c * (g.n[2] + 1) + t) * g.n[1] + y) * g.n[0] + x;
+}
+
+template <int M, typename R>
+__device__ __forceinline__ R wave_speed(R vn, R lf, R la, R ls) {
+    if constexpr (M == 0) return vn - lf;
+    else if constexpr (M == 1) return vn - la;
+    else if constexpr (M == 2) return vn - ls;
+    else if constexpr (M == 3) return vn;
+    else if constexpr (M == 4) return vn + ls;
+    else if constexpr (M == 5) return vn + la;
+    else return vn + lf;
+}
+
+// One wave of the characteristic flux: project the 6-cell stencil on L row M,
+// WENO5 with LLF splitting, add R column M times the wave flux to dF
+// (src/ssprk.cpp:847-943, waves added in order 0..6).
+template <int M, typename R>
+__device__ __forceinline__ void wave_flux(R gam, const Eig<R>& e, const R (&U)[6][7], const R (&F)[6][7],
+                                          const R (&spd)[6][4], R (&dF)[7]) {
+    R amx = R(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) amx = rmax(amx, rabs(wave_speed<M>(spd[k][0], spd[k][1], spd[k][2], spd[k][3])));
+    R L[7];
+    l_row<M>(gam, e, L);
+    R Fsk[6], qsk[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        Fsk[k] = L[0] * F[k][0] + L[1] * F[k][1] + L[2] * F[k][2] + L[3] * F[k][3] + L[4] * F[k][4] +
+                 L[5] * F[k][5] + L[6] * F[k][6];
+        qsk[k] = L[0] * U[k][0] + L[1] * U[k][1] + L[2] * U[k][2] + L[3] * U[k][3] + L[4] * U[k][4] +
+                 L[5] * U[k][5] + L[6] * U[k][6];
+    }
+    const R Fs = weno_wave<R>(Fsk, qsk, amx);
+    R Rc[7];
+    r_col<M>(gam, e, Rc);
+#pragma unroll
+    for (int r = 0; r < 7; ++r) dF[r] += Rc[r] * Fs;
+}
+
+template <typename R, int DIR>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs<R> A) {
+    using T = SweepTile<DIR>;
+    __shared__ R lds[C_NFIELD][T::CELLS];
+    const Geom& g = A.g;
+    const int nd = g.n[DIR];
+    const int per = g.per[DIR];
+    const int tbeg = per ? 1 : 0;  // periodic: interface 0 is the image of n
+    const int tid = threadIdx.x;
+
+    // block origin
+    int t0, x0 = 0, y0 = 0, z0 = 0;
+    if constexpr (DIR == 0) {
+        t0 = tbeg + blockIdx.x * T::LANES;
+        y0 = blockIdx.y * T::ROWS;
+        z0 = blockIdx.z;
+    } else {
+        x0 = blockIdx.x * T::LANES;
+        t0 = tbeg + blockIdx.y * T::ROWS;
+        if constexpr (DIR == 1) z0 = blockIdx.z;
+        else y0 = blockIdx.z;
+    }
+
+    // ---- phase 1: cell values of the tile (stencil halo included) into LDS
+    for (int s = tid; s < T::CELLS; s += 256) {
+        int cx, cy, cz, along;
+        if constexpr (DIR == 0) {
+            const int row = s / (T::LANES + 5), c = s % (T::LANES + 5);
+            along = t0 - 3 + c;
+            cy = y0 + row; cz = z0; cx = 0;
+            if (cy >= g.n[1]) cy = g.n[1] - 1;
+        } else {
+            const int row = s / T::LANES, lane = s % T::LANES;
+            along = t0 - 3 + row;
+            cx = x0 + lane;
+            if (cx >= g.n[0]) cx = g.n[0] - 1;
+            if constexpr (DIR == 1) { cz = z0; cy = 0; }
+            else { cy = y0; cz = 0; }
+        }
+        if (along > nd + 2) along = nd + 2;
+        const int a = bmap(along, nd, per);
+        if constexpr (DIR == 0) cx = a;
+        else if constexpr (DIR == 1) cy = a;
+        else cz = a;
+        const long long idx = ((long long)cz * g.n[1] + cy) * g.n[0] + cx;
+        double qs[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) qs[m] = A.q[(long long)kPerm[DIR][m] * g.ncell + idx];
+        R out[C_NFIELD];
+        cell_values<R>(qs, A.ph.gam, A.ph.rhomin, A.ph.pgmin, out);
+#pragma unroll
+        for (int f = 0; f < C_NFIELD; ++f) lds[f][s] = out[f];
+    }
+    __syncthreads();
+
+    // ---- phase 2: one interface per thread
+    int lane, row, t, x, y, z;
+    if constexpr (DIR == 0) {
+        lane = tid % T::LANES; row = tid / T::LANES;
+        t = t0 + lane; x = 0; y = y0 + row; z = z0;
+        if (t > nd || y >= g.n[1]) return;
+    } else {
+        lane = tid % T::LANES; row = tid / T::LANES;
+        t = t0 + row; x = x0 + lane;
+        if constexpr (DIR == 1) { y = 0; z = z0; }
+        else { y = y0; z = 0; }
+        if (t > nd || x >= g.n[0]) return;
+    }
+    auto slot = [&](int k) -> int {
+        if constexpr (DIR == 0) return row * (T::LANES + 5) + lane + k;
+        else return (row + k) * T::LANES + lane;
+    };
+
+    R U[6][7], F[6][7], spd[6][4];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int s = slot(k);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            U[k][j] = lds[C_RHO + j][s];
+            F[k][j] = lds[C_F0 + j][s];
+        }
+        spd[k][0] = lds[C_VN][s]; spd[k][1] = lds[C_LF][s]; spd[k][2] = lds[C_LA][s]; spd[k][3] = lds[C_LS][s];
+    }
+    // interface state from cells k = 2, 3: (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H)
+    R u0[9], u1[9];
+    {
+        const int s0 = slot(2), s1 = slot(3);
+        const int fl[9] = {C_RHO, C_VN, C_VT1, C_VT2, C_BN, C_BT1, C_BT2, C_P, C_H};
+#pragma unroll
+        for (int v = 0; v < 9; ++v) { u0[v] = lds[fl[v]][s0]; u1[v] = lds[fl[v]][s1]; }
+    }
+    const R gam = R(A.ph.gam);
+    Eig<R> e;
+    eig_scalars<R>(gam, u0, u1, e);
+    R dF[7] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+    wave_flux<0, R>(gam, e, U, F, spd, dF);
+    wave_flux<1, R>(gam, e, U, F, spd, dF);
+    wave_flux<2, R>(gam, e, U, F, spd, dF);
+    wave_flux<3, R>(gam, e, U, F, spd, dF);
+    wave_flux<4, R>(gam, e, U, F, spd, dF);
+    wave_flux<5, R>(gam, e, U, F, spd, dF);
+    wave_flux<6, R>(gam, e, U, F, spd, dF);
+
+#pragma unroll
+    for (int c = 0; c < 7; ++c) A.dF[dF_index<DIR>(g, c, t, x, y, z)] = dF[c];
+
+    // CT flux terms, src/ssprk.cpp:967-994 (cells k = 1..4; the -0.0 weights
+    // are kept as written)
+    if (t >= 1) {
+        R bnv1[4], bnv2[4];
+#pragma unroll
+        for (int k = 1; k <= 4; ++k) {
+            const int s = slot(k);
+            const R bn = lds[C_BN][s];
+            bnv1[k - 1] = bn * lds[C_VT1][s];
+            bnv2[k - 1] = bn * lds[C_VT2][s];
+        }
+        const R ftmp1 = div_c<R>(R(-0.0) * bnv1[0] + R(9) * bnv1[1] + R(9) * bnv1[2] - R(0) * bnv1[3], R(18),
+                                 R(1.0 / 18.0));
+        const R ftmp2 = div_c<R>(R(-0.0) * bnv2[0] + R(9) * bnv2[1] + R(9) * bnv2[2] - R(0) * bnv2[3], R(18),
+                                 R(1.0 / 18.0));
+        long long ci;
+        if constexpr (DIR == 0) ci = ((long long)z * g.n[1] + y) * g.n[0] + (t - 1);
+        else if constexpr (DIR == 1) ci = ((long long)z * g.n[1] + (t - 1)) * g.n[0] + x;
+        else ci = ((long long)(t - 1) * g.n[1] + y) * g.n[0] + x;
+        A.ct_a[ci] = dF[4] + ftmp1;
+        A.ct_b[ci] = dF[5] + ftmp2;
+    }
+}
+
+// ---------------------------------------------------------------- CT kernels
+struct CtFlags {
+    int use_x, use_y, use_z, use_Ox, use_Oy, use_Oz, active;
+};
+
+template <typename R>
+struct EmfArgs {
+    const R *fsy, *fsz, *gsx, *gsz, *hsx, *hsy;
+    R *Ox2, *Oy2, *Oz2;
+    Geom g;
+    CtFlags f;
+};
+
+__device__ __forceinline__ void cell_of(const Geom& g, long long i, int& x, int& y, int& z) {
+    x = (int)(i % g.n[0]);
+    const long long r = i / g.n[0];
+    y = (int)(r % g.n[1]);
+    z = (int)(r / g.n[1]);
+}
+__device__ __forceinline__ long long lin(const Geom& g, int x, int y, int z) {
+    return ((long long)z * g.n[1] + y) * g.n[0] + x;
+}
+
+// 4-point interpolation of src/fluxct.cpp:815-820: (-a + 9b + 9c - d)/16.
+template <typename R>
+__device__ __forceinline__ R i4(R a, R b, R c, R d) {
+    return div_c<R>(-a + R(9) * b + R(9) * c - d, R(16), R(1.0 / 16.0));
+}
+
+// fluxct_build_O2, src/fluxct.cpp:809-821, on active cells
+template <typename R>
+__global__ __launch_bounds__(256) void k_emf2(EmfArgs<R> A) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const Geom& g = A.g;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    cell_of(g, i, x, y, z);
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    if (A.f.use_Ox) {
+        const int xm = bmap(x - 1, nx, g.per[0]), xp = bmap(x + 1, nx, g.per[0]), xq = bmap(x + 2, nx, g.per[0]);
+        const int ym = bmap(y - 1, ny, g.per[1]), yp = bmap(y + 1, ny, g.per[1]), yq = bmap(y + 2, ny, g.per[1]);
+        A.Ox2[i] = i4<R>(A.gsx[lin(g, xm, y, z)], A.gsx[i], A.gsx[lin(g, xp, y, z)], A.gsx[lin(g, xq, y, z)]) -
+                   i4<R>(A.fsy[lin(g, x, ym, z)], A.fsy[i], A.fsy[lin(g, x, yp, z)], A.fsy[lin(g, x, yq, z)]);
+    }
+    if (A.f.use_Oy) {
+        const int ym = bmap(y - 1, ny, g.per[1]), yp = bmap(y + 1, ny, g.per[1]), yq = bmap(y + 2, ny, g.per[1]);
+        const int zm = bmap(z - 1, nz, g.per[2]), zp = bmap(z + 1, nz, g.per[2]), zq = bmap(z + 2, nz, g.per[2]);
+        A.Oy2[i] = i4<R>(A.hsy[lin(g, x, ym, z)], A.hsy[i], A.hsy[lin(g, x, yp, z)], A.hsy[lin(g, x, yq, z)]) -
+                   i4<R>(A.gsz[lin(g, x, y, zm)], A.gsz[i], A.gsz[lin(g, x, y, zp)], A.gsz[lin(g, x, y, zq)]);
+    }
+    if (A.f.use_Oz) {
+        const int zm = bmap(z - 1, nz, g.per[2]), zp = bmap(z + 1, nz, g.per[2]), zq = bmap(z + 2, nz, g.per[2]);
+        const int xm = bmap(x - 1, nx, g.per[0]), xp = bmap(x + 1, nx, g.per[0]), xq = bmap(x + 2, nx, g.per[0]);
+        A.Oz2[i] = i4<R>(A.fsz[lin(g, x, y, zm)], A.fsz[i], A.fsz[lin(g, x, y, zp)], A.fsz[lin(g, x, y, zq)]) -
+                   i4<R>(A.hsx[lin(g, xm, y, z)], A.hsx[i], A.hsx[lin(g, xp, y, z)], A.hsx[lin(g, xq, y, z)]);
+    }
+}
+
+// (a - 2c + b)/24 of src/fluxct.cpp:834-839
+template <typename R>
+__device__ __forceinline__ R lap24(R m, R c, R p) {
+    return div_c<R>(m - R(2) * c + p, R(24), R(1.0 / 24.0));
+}
+
+template <typename R>
+struct SmoothArgs {
+    const R *Ox2, *Oy2, *Oz2;
+    R *Ox, *Oy, *Oz;
+    Geom g;
+    CtFlags f;
+};
+
+// fluxct_smooth_O, src/fluxct.cpp:828-840
+template <typename R>
+__global__ __launch_bounds__(256) void k_emf(SmoothArgs<R> A) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const Geom& g = A.g;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    cell_of(g, i, x, y, z);
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    if (A.f.use_Ox) {
+        const R o = A.Ox2[i];
+        A.Ox[i] = o + lap24<R>(A.Ox2[lin(g, bmap(x - 1, nx, g.per[0]), y, z)], o, A.Ox2[lin(g, bmap(x + 1, nx, g.per[0]), y, z)]) +
+                  lap24<R>(A.Ox2[lin(g, x, bmap(y - 1, ny, g.per[1]), z)], o, A.Ox2[lin(g, x, bmap(y + 1, ny, g.per[1]), z)]);
+    }
+    if (A.f.use_Oy) {
+        const R o = A.Oy2[i];
+        A.Oy[i] = o + lap24<R>(A.Oy2[lin(g, x, bmap(y - 1, ny, g.per[1]), z)], o, A.Oy2[lin(g, x, bmap(y + 1, ny, g.per[1]), z)]) +
+                  lap24<R>(A.Oy2[lin(g, x, y, bmap(z - 1, nz, g.per[2]))], o, A.Oy2[lin(g, x, y, bmap(z + 1, nz, g.per[2]))]);
+    }
+    if (A.f.use_Oz) {
+        const R o = A.Oz2[i];
+        A.Oz[i] = o + lap24<R>(A.Oz2[lin(g, bmap(x - 1, nx, g.per[0]), y, z)], o, A.Oz2[lin(g, bmap(x + 1, nx, g.per[0]), y, z)]) +
+                  lap24<R>(A.Oz2[lin(g, x, y, bmap(z - 1, nz, g.per[2]))], o, A.Oz2[lin(g, x, y, bmap(z + 1, nz, g.per[2]))]);
+    }
+}
+
+struct StageCoef {
+    double k1, k3, c4, c5;  // c4 = k4/k1, c5 = k5/k1 (stage 5 only)
+    double dtdx, dtdy, dtdz;
+    int stage5;
+};
+
+template <typename R>
+struct FaceArgs {
+    const R *Ox, *Oy, *Oz;
+    const double* b0;   // stage-0 faces (3 arrays of ncell)
+    const double* b2;   // stage 2 (stage 5 only)
+    const double* b3;   // stage 3 (stage 5 only)
+    const double* bp;   // previous stage
+    double* bo;         // this stage
+    Geom g;
+    CtFlags f;
+    StageCoef s;
+};
+
+// Staggered 4th-order difference aa*(o0-om1) + bb*(op1-om2) + cc*(op2-om3),
+// aa = 9/8, bb = -1/24, cc = 0 (src/fluxct.cpp:848, 861-874), in FP64.
+__device__ __forceinline__ double d4(double om3, double om2, double om1, double o0, double op1, double op2) {
+    const double aa = 9.0 / 8.0, bb = -1.0 / 24.0, cc = 0.0;
+    return aa * (o0 - om1) + bb * (op1 - om2) + cc * (op2 - om3);
+}
+
+// fluxct_update_bfield, src/fluxct.cpp:850-877, with the stage initialisation
+// of copy_bfield_stage / prepare_bfield_stage5 (111-135) folded in.
+template <typename R>
+__global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const Geom& g = A.g;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    cell_of(g, i, x, y, z);
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const CtFlags& f = A.f;
+    const StageCoef& s = A.s;
+    double binit[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const long long o = (long long)c * g.ncell + i;
+        binit[c] = s.stage5 ? A.b0[o] + s.c4 * A.b2[o] + s.c5 * A.b3[o] : A.b0[o];
+    }
+    auto Oat = [&](const R* O, int xx, int yy, int zz) -> double { return (double)O[lin(g, xx, yy, zz)]; };
+    auto dy_of = [&](const R* O) -> double {
+        const int p = g.per[1];
+        return d4(Oat(O, x, bmap(y - 3, ny, p), z), Oat(O, x, bmap(y - 2, ny, p), z), Oat(O, x, bmap(y - 1, ny, p), z),
+                  Oat(O, x, y, z), Oat(O, x, bmap(y + 1, ny, p), z), Oat(O, x, bmap(y + 2, ny, p), z));
+    };
+    auto dz_of = [&](const R* O) -> double {
+        const int p = g.per[2];
+        return d4(Oat(O, x, y, bmap(z - 3, nz, p)), Oat(O, x, y, bmap(z - 2, nz, p)), Oat(O, x, y, bmap(z - 1, nz, p)),
+                  Oat(O, x, y, z), Oat(O, x, y, bmap(z + 1, nz, p)), Oat(O, x, y, bmap(z + 2, nz, p)));
+    };
+    auto dx_of = [&](const R* O) -> double {
+        const int p = g.per[0];
+        return d4(Oat(O, bmap(x - 3, nx, p), y, z), Oat(O, bmap(x - 2, nx, p), y, z), Oat(O, bmap(x - 1, nx, p), y, z),
+                  Oat(O, x, y, z), Oat(O, bmap(x + 1, nx, p), y, z), Oat(O, bmap(x + 2, nx, p), y, z));
+    };
+    double bx = binit[0], by = binit[1], bz = binit[2];
+    if (f.active) {
+        if (f.use_Ox || f.use_Oz) {
+            double dOx = 0.0, dOz = 0.0;
+            if (f.use_Ox) dOx = dy_of(A.Ox);
+            if (f.use_Oz) dOz = dz_of(A.Oz);
+            bx = s.k1 * binit[0] + s.k3 * A.bp[i] - s.dtdy * dOx + s.dtdz * dOz;
+        }
+        if (f.use_Oy || f.use_Ox) {
+            double dOy = 0.0, dOx = 0.0;
+            if (f.use_Oy) dOy = dz_of(A.Oy);
+            if (f.use_Ox) dOx = dx_of(A.Ox);
+            by = s.k1 * binit[1] + s.k3 * A.bp[g.ncell + i] - s.dtdz * dOy + s.dtdx * dOx;
+        }
+        if (f.use_Oz || f.use_Oy) {
+            double dOz = 0.0, dOy = 0.0;
+            if (f.use_Oz) dOz = dx_of(A.Oz);
+            if (f.use_Oy) dOy = dy_of(A.Oy);
+            bz = s.k1 * binit[2] + s.k3 * A.bp[2 * g.ncell + i] - s.dtdx * dOz + s.dtdy * dOy;
+        }
+    }
+    A.bo[i] = bx;
+    A.bo[g.ncell + i] = by;
+    A.bo[2 * g.ncell + i] = bz;
+}
+
+template <typename R>
+struct CombineArgs {
+    const double* q0;
+    const double* q2;   // stage 5 only
+    const double* q3;   // stage 5 only
+    const double* qp;   // previous stage
+    double* qo;         // this stage
+    const double* bo;   // faces of this stage
+    const R *dFx, *dFy, *dFz;
+    Geom g;
+    CtFlags f;
+    StageCoef s;
+    Phys ph;
+};
+
+// 6th-order face -> centre interpolation of src/fluxct.cpp:891-896:
+// (3 b(-3) - 25 b(-2) + 150 b(-1) + 150 b(0) - 25 b(1) + 3 b(2)) / 256
+__device__ __forceinline__ double i6(double a, double b, double c, double d, double e, double f) {
+    return div_c<double>(3.0 * a - 25.0 * b + 150.0 * c + 150.0 * d - 25.0 * e + 3.0 * f, 256.0, 1.0 / 256.0);
+}
+
+// Stage combination: x_sweep's store q = k1*q + k3*q_prev + dX
+// (src/ssprk.cpp:415-430), then += dY, += dZ (1075-1094, 1166-1185); face ->
+// centre B (fluxct.cpp:882-897); prot (prot.cpp:10-51).
+template <typename R>
+__global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const Geom& g = A.g;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    cell_of(g, i, x, y, z);
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const StageCoef& s = A.s;
+    double v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const long long o = (long long)m * g.ncell + i;
+        v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
+    }
+    // slot of each q variable in the dF of each sweep (-1: the normal field)
+    // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
+    // y: rho->0, Mx->3, My->1, Mz->2, Bx->5, By->-, Bz->4, E->6
+    // z: rho->0, Mx->2, My->3, Mz->1, Bx->4, By->5, Bz->-, E->6
+    constexpr int sx[8] = {0, 1, 2, 3, -1, 4, 5, 6};
+    constexpr int sy[8] = {0, 3, 1, 2, 5, -1, 4, 6};
+    constexpr int sz[8] = {0, 2, 3, 1, 4, 5, -1, 6};
+    if (nx > 1) {
+        const int tl = (x == 0 && g.per[0]) ? nx : x, tr = x + 1;
+        const double dt = s.dtdx;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            double d = 0.0;
+            if (sx[m] >= 0)
+                d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
+            v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
+        }
+    }
+    if (ny > 1) {
+        const int tl = (y == 0 && g.per[1]) ? ny : y, tr = y + 1;
+        const double dt = s.dtdy;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            double d = 0.0;
+            if (sy[m] >= 0)
+                d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] - (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
+            v[m] = v[m] + d;
+        }
+    }
+    if (nz > 1) {
+        const int tl = (z == 0 && g.per[2]) ? nz : z, tr = z + 1;
+        const double dt = s.dtdz;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            double d = 0.0;
+            if (sz[m] >= 0)
+                d = -dt * ((double)A.dFz[dF_index<2>(g, sz[m], tr, x, y, z)] - (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, z)]);
+            v[m] = v[m] + d;
+        }
+    }
+    if (A.f.active) {
+        const double* bx = A.bo;
+        const double* by = A.bo + g.ncell;
+        const double* bz = A.bo + 2 * g.ncell;
+        if (A.f.use_x) {
+            const int p = g.per[0];
+            v[4] = i6(bx[lin(g, bmap(x - 3, nx, p), y, z)], bx[lin(g, bmap(x - 2, nx, p), y, z)], bx[lin(g, bmap(x - 1, nx, p), y, z)],
+                      bx[i], bx[lin(g, bmap(x + 1, nx, p), y, z)], bx[lin(g, bmap(x + 2, nx, p), y, z)]);
+        } else {
+            v[4] = bx[i];
+        }
+        if (A.f.use_y) {
+            const int p = g.per[1];
+            v[5] = i6(by[lin(g, x, bmap(y - 3, ny, p), z)], by[lin(g, x, bmap(y - 2, ny, p), z)], by[lin(g, x, bmap(y - 1, ny, p), z)],
+                      by[i], by[lin(g, x, bmap(y + 1, ny, p), z)], by[lin(g, x, bmap(y + 2, ny, p), z)]);
+        } else {
+            v[5] = by[i];
+        }
+        if (A.f.use_z) {
+            const int p = g.per[2];
+            v[6] = i6(bz[lin(g, x, y, bmap(z - 3, nz, p))], bz[lin(g, x, y, bmap(z - 2, nz, p))], bz[lin(g, x, y, bmap(z - 1, nz, p))],
+                      bz[i], bz[lin(g, x, y, bmap(z + 1, nz, p))], bz[lin(g, x, y, bmap(z + 2, nz, p))]);
+        } else {
+            v[6] = bz[i];
+        }
+    }
+    // prot
+    const double gam = A.ph.gam;
+    double rho = fmax(v[0], A.ph.rhomin);
+    const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
+    const double vv2 = vx * vx + vy * vy + vz * vz;
+    const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
+    double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
+    pg = fmax(pg, A.ph.pgmin);
+    v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
+    v[0] = rho;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
+}
+
+// ---------------------------------------------------------------- CFL (tstep)
+// Per-direction maximum of |v_d| + c_fast,d over active cells of q(0)
+// (src/tstep.cpp:28-83).  Non-negative doubles order like their bit patterns,
+// so the block maxima are combined with a 64-bit atomicMax.
+__global__ __launch_bounds__(256) void k_cfl(const double* q, Geom g, Phys ph, unsigned long long* out) {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    const double gam = ph.gam;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < g.ncell; i += (long long)gridDim.x * 256) {
+        const double DD = q[i], Mx = q[g.ncell + i], My = q[2 * g.ncell + i], Mz = q[3 * g.ncell + i];
+        const double Bx = q[4 * g.ncell + i], By = q[5 * g.ncell + i], Bz = q[6 * g.ncell + i], EE = q[7 * g.ncell + i];
+        const double rho = fmax(DD, ph.rhomin);
+        const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
+        const double vv2 = vx * vx + vy * vy + vz * vz;
+        const double BB2 = Bx * Bx + By * By + Bz * Bz;
+        const double pg = fmax(ph.pgmin, (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2)));
+        const double bbn2 = BB2 / rho;
+        const double cs2 = fmax(0.0, gam * pg / rho);
+        const double bnx2 = Bx * Bx / rho, bny2 = By * By / rho, bnz2 = Bz * Bz / rho;
+        const double rootx = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+        const double rooty = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bny2 * cs2));
+        const double rootz = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnz2 * cs2));
+        const double lfx = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootx)));
+        const double lfy = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rooty)));
+        const double lfz = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootz)));
+        m0 = fmax(m0, fabs(vx) + lfx);
+        m1 = fmax(m1, fabs(vy) + lfy);
+        m2 = fmax(m2, fabs(vz) + lfz);
+    }
+    // wavefront reduction (64 lanes), then across the 4 wavefronts
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        m0 = fmax(m0, __shfl_xor(m0, off, 64));
+        m1 = fmax(m1, __shfl_xor(m1, off, 64));
+        m2 = fmax(m2, __shfl_xor(m2, off, 64));
+    }
+    __shared__ double red[3][4];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { red[0][w] = m0; red[1][w] = m1; red[2][w] = m2; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int d = threadIdx.x;
+        const double r = fmax(fmax(red[d][0], red[d][1]), fmax(red[d][2], red[d][3]));
+        atomicMax(&out[d], (unsigned long long)__double_as_longlong(r));
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// initialize_bfield_from_q, src/fluxct.cpp:65-107, on active faces
+__global__ __launch_bounds__(256) void k_init_faces(const double* q, double* b, Geom g) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    cell_of(g, i, x, y, z);
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const double* Bx = q + 4 * g.ncell;
+    const double* By = q + 5 * g.ncell;
+    const double* Bz = q + 6 * g.ncell;
+    const int px = g.per[0], py = g.per[1], pz = g.per[2];
+    b[i] = (-Bx[lin(g, bmap(x - 1, nx, px), y, z)] + 9.0 * Bx[i] + 9.0 * Bx[lin(g, bmap(x + 1, nx, px), y, z)] -
+            Bx[lin(g, bmap(x + 2, nx, px), y, z)]) / 16.0;
+    b[g.ncell + i] = (-By[lin(g, x, bmap(y - 1, ny, py), z)] + 9.0 * By[i] + 9.0 * By[lin(g, x, bmap(y + 1, ny, py), z)] -
+                      By[lin(g, x, bmap(y + 2, ny, py), z)]) / 16.0;
+    b[2 * g.ncell + i] = (-Bz[lin(g, x, y, bmap(z - 1, nz, pz))] + 9.0 * Bz[i] + 9.0 * Bz[lin(g, x, y, bmap(z + 1, nz, pz))] -
+                          Bz[lin(g, x, y, bmap(z + 2, nz, pz))]) / 16.0;
+}
+
+}  // namespace howmhd

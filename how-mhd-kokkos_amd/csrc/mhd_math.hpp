@@ -1,0 +1,415 @@
+// Device-side MHD arithmetic of the HIP time step.
+//
+// Every expression follows the reference's evaluation order (left-to-right
+// sums and products, the same divisions), so that the library built with
+// -ffp-contract=off (HOWMHD_EXACT=1) rounds each value exactly as the
+// reference does.  The product build (HOWMHD_EXACT=0) lets the compiler fuse
+// multiply-adds and replaces divisions by constants and the WENO weight
+// normalisation with reciprocal forms.
+//
+// Reference: src/ssprk.cpp 458-516 (primitives), 538-595 (cell eigenvalues and
+// fluxes), 602-818 (interface eigenvectors), 841-945 (LLF-WENO5 flux),
+// 967-994 (CT flux terms).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#ifndef HOWMHD_EXACT
+#define HOWMHD_EXACT 0
+#endif
+
+namespace howmhd {
+
+template <typename R>
+__device__ __forceinline__ R sign1(R x) { return (x >= R(0)) ? R(1) : R(-1); }
+
+template <typename R>
+__device__ __forceinline__ R rmax(R a, R b) { return fmax(a, b); }
+template <>
+__device__ __forceinline__ float rmax<float>(float a, float b) { return fmaxf(a, b); }
+
+template <typename R>
+__device__ __forceinline__ R rsqrt_(R x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float rsqrt_<float>(float x) { return sqrtf(x); }
+
+template <typename R>
+__device__ __forceinline__ R rabs(R x) { return fabs(x); }
+template <>
+__device__ __forceinline__ float rabs<float>(float x) { return fabsf(x); }
+
+// x / c for a compile-time constant c.  Exact build: a true division, as the
+// reference writes it.  Product build: multiplication by the rounded
+// reciprocal (differs by at most one rounding).
+template <typename R>
+__device__ __forceinline__ R div_c(R x, R c, R inv_c) {
+#if HOWMHD_EXACT
+    (void)inv_c;
+    return x / c;
+#else
+    (void)c;
+    return x * inv_c;
+#endif
+}
+
+// ----------------------------------------------------------------- cell data
+// Per-cell values a sweep needs, in sweep order (n = normal, t1/t2 =
+// transverse): the conserved values after the primitive floors, the physical
+// flux, the wave-speed magnitudes and the primitives the interface averages
+// use.  Field indices of the LDS tile:
+enum CellField {
+    C_RHO = 0, C_MN, C_MT1, C_MT2, C_BT1, C_BT2, C_E,      // U (projection input)
+    C_F0, C_F1, C_F2, C_F3, C_F4, C_F5, C_F6,              // F
+    C_VN, C_LF, C_LA, C_LS,                                // eigenvalue inputs
+    C_VT1, C_VT2, C_BN, C_P, C_H,                          // interface averages
+    C_NFIELD
+};
+
+// primit_x_all + eigenst_x_all centre for one cell (src/ssprk.cpp:470-593).
+// Input: conserved in sweep order, FP64.  The primitive conversion, the
+// floors and E -> p stay in FP64; the outputs are rounded to R.
+template <typename R>
+struct CellOut {
+    R f[C_NFIELD];
+};
+
+template <typename R>
+__device__ __forceinline__ void cell_values(const double* qs, double gam, double rhomin, double pgmin,
+                                            R* out) {
+    const double DD = qs[0], Mx = qs[1], My = qs[2], Mz = qs[3];
+    const double Bx = qs[4], By = qs[5], Bz = qs[6];
+    double EE = qs[7];
+    double rho = DD;
+    const double vx = Mx / rho;
+    const double vy = My / rho;
+    const double vz = Mz / rho;
+    const double vv2 = vx * vx + vy * vy + vz * vz;
+    const double BB2 = Bx * Bx + By * By + Bz * Bz;
+    double pg = (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2));
+    double Dq = DD;
+    if (rho < rhomin || pg < pgmin) {
+        rho = fmax(rhomin, rho);
+        pg = fmax(pgmin, pg);
+        EE = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
+        Dq = rho;
+    }
+    const double HH = (EE + pg) / rho;
+
+    // eigenst centre, in R
+    const R rrho = R(rho), rvx = R(vx), rvy = R(vy), rvz = R(vz);
+    const R rBx = R(Bx), rBy = R(By), rBz = R(Bz), rpg = R(pg);
+    const R rDD = R(Dq), rMx = R(Mx), rMy = R(My), rMz = R(Mz), rEE = R(EE);
+    const R rgam = R(gam);
+    const R rBB2 = rBx * rBx + rBy * rBy + rBz * rBz;
+    const R pt = rpg + R(0.5) * rBB2;
+    const R vdB = rvx * rBx + rvy * rBy + rvz * rBz;
+    const R bbn2 = rBB2 / rrho;
+    const R bnx2 = rBx * rBx / rrho;
+    const R cs2 = rmax(R(0), rgam * rabs(rpg / rrho));
+    const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
+    const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
+    const R la = rsqrt_(rmax(R(0), bnx2));
+    const R ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
+
+    out[C_RHO] = rDD;  out[C_MN] = rMx;  out[C_MT1] = rMy;  out[C_MT2] = rMz;
+    out[C_BT1] = rBy;  out[C_BT2] = rBz; out[C_E] = rEE;
+    out[C_F0] = rDD * rvx;
+    out[C_F1] = rMx * rvx + pt - rBx * rBx;
+    out[C_F2] = rMy * rvx - rBx * rBy;
+    out[C_F3] = rMz * rvx - rBx * rBz;
+    out[C_F4] = rBy * rvx - rBx * rvy;
+    out[C_F5] = rBz * rvx - rBx * rvz;
+    out[C_F6] = (rEE + pt) * rvx - rBx * vdB;
+    out[C_VN] = rvx;  out[C_LF] = lf;  out[C_LA] = la;  out[C_LS] = ls;
+    out[C_VT1] = rvy; out[C_VT2] = rvz; out[C_BN] = rBx; out[C_P] = rpg; out[C_H] = R(HH);
+    (void)rrho;
+}
+
+// ------------------------------------------------------ interface eigensystem
+// Scalars of the interface eigensystem (src/ssprk.cpp:610-659).  Rows of L
+// and columns of R are built from them one wave at a time.
+template <typename R>
+struct Eig {
+    R vx, vy, vz, By, Bz, vv2, cs, cs2, lf, ls, af, as, bty, btz, sgnBx, sqrtrho, invsqrtrho, sgnBt;
+    bool slow_flip;  // cs >= la: waves 2 and 4 take the sign of Bt; else waves 0 and 6
+};
+
+template <typename R>
+__device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig<R>& e) {
+    // u = (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H) of the two cells
+    const R rho = R(0.5) * (u0[0] + u1[0]);
+    const R vx = R(0.5) * (u0[1] + u1[1]);
+    const R vy = R(0.5) * (u0[2] + u1[2]);
+    const R vz = R(0.5) * (u0[3] + u1[3]);
+    const R Bx = R(0.5) * (u0[4] + u1[4]);
+    const R By = R(0.5) * (u0[5] + u1[5]);
+    const R Bz = R(0.5) * (u0[6] + u1[6]);
+    const R HH = R(0.5) * (u0[8] + u1[8]);
+    const R vv2 = vx * vx + vy * vy + vz * vz;
+    const R BB2 = Bx * Bx + By * By + Bz * Bz;
+    const R bbn2 = BB2 / rho;
+    const R bnx2 = Bx * Bx / rho;
+    const R cs2 = (gam - R(1)) * (HH - R(0.5) * (vv2 + bbn2));
+    const R cs = rsqrt_(rmax(R(0), cs2));
+    const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
+    const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
+    const R la = rsqrt_(rmax(R(0), bnx2));
+    const R ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
+    const R Bt2 = By * By + Bz * Bz;
+    R bty, btz;
+    if (Bt2 >= R(1.0e-30)) {
+        bty = By / rsqrt_(Bt2);
+        btz = Bz / rsqrt_(Bt2);
+    } else {
+        bty = R(1) / rsqrt_(R(2));
+        btz = R(1) / rsqrt_(R(2));
+    }
+    R af, as;
+    if ((lf * lf - ls * ls) >= R(1.0e-30)) {
+        af = rsqrt_(rmax(R(0), cs2 - ls * ls)) / rsqrt_(lf * lf - ls * ls);
+        as = rsqrt_(rmax(R(0), lf * lf - cs2)) / rsqrt_(lf * lf - ls * ls);
+    } else {
+        af = R(1);
+        as = R(1);
+    }
+    const R sqrtrho = rsqrt_(rho);
+    e.vx = vx; e.vy = vy; e.vz = vz; e.By = By; e.Bz = Bz; e.vv2 = vv2;
+    e.cs = cs; e.cs2 = cs2; e.lf = lf; e.ls = ls; e.af = af; e.as = as;
+    e.bty = bty; e.btz = btz; e.sgnBx = sign1(Bx);
+    e.sqrtrho = sqrtrho;
+    e.invsqrtrho = R(1) / sqrtrho;
+    e.sgnBt = (By != R(0)) ? sign1(By) : sign1(Bz);
+    e.slow_flip = (cs >= la);
+}
+
+// Row m of L after normalisation and sign flip (src/ssprk.cpp:662-726,
+// 789-806).
+template <int M, typename R>
+__device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
+    const R gam0 = R(1) - gam;
+    const R gam1 = R(0.5) * (gam - R(1));
+    const R af = e.af, as = e.as, lf = e.lf, ls = e.ls, cs = e.cs, cs2 = e.cs2;
+    const R vx = e.vx, vy = e.vy, vz = e.vz, By = e.By, Bz = e.Bz, vv2 = e.vv2;
+    const R bty = e.bty, btz = e.btz, sgnBx = e.sgnBx, sqrtrho = e.sqrtrho;
+    if constexpr (M == 0) {
+        L[0] = af * (gam1 * vv2 + lf * vx) - as * ls * (bty * vy + btz * vz) * sgnBx;
+        L[1] = af * (gam0 * vx - lf);
+        L[2] = gam0 * af * vy + as * ls * bty * sgnBx;
+        L[3] = gam0 * af * vz + as * ls * btz * sgnBx;
+        L[4] = gam0 * af * By + cs * as * bty * sqrtrho;
+        L[5] = gam0 * af * Bz + cs * as * btz * sqrtrho;
+        L[6] = -gam0 * af;
+    } else if constexpr (M == 1) {
+        L[0] = btz * vy - bty * vz;
+        L[1] = R(0);
+        L[2] = -btz;
+        L[3] = bty;
+        L[4] = -btz * sgnBx * sqrtrho;
+        L[5] = bty * sgnBx * sqrtrho;
+        L[6] = R(0);
+    } else if constexpr (M == 2) {
+        L[0] = as * (gam1 * vv2 + ls * vx) + af * lf * (bty * vy + btz * vz) * sgnBx;
+        L[1] = gam0 * as * vx - as * ls;
+        L[2] = gam0 * as * vy - af * lf * bty * sgnBx;
+        L[3] = gam0 * as * vz - af * lf * btz * sgnBx;
+        L[4] = gam0 * as * By - cs * af * bty * sqrtrho;
+        L[5] = gam0 * as * Bz - cs * af * btz * sqrtrho;
+        L[6] = -gam0 * as;
+    } else if constexpr (M == 3) {
+        L[0] = -cs2 / gam0 - R(0.5) * vv2;
+        L[1] = vx;
+        L[2] = vy;
+        L[3] = vz;
+        L[4] = By;
+        L[5] = Bz;
+        L[6] = R(-1);
+    } else if constexpr (M == 4) {
+        L[0] = as * (gam1 * vv2 - ls * vx) - af * lf * (bty * vy + btz * vz) * sgnBx;
+        L[1] = as * (gam0 * vx + ls);
+        L[2] = gam0 * as * vy + af * lf * bty * sgnBx;
+        L[3] = gam0 * as * vz + af * lf * btz * sgnBx;
+        L[4] = gam0 * as * By - cs * af * bty * sqrtrho;
+        L[5] = gam0 * as * Bz - cs * af * btz * sqrtrho;
+        L[6] = -gam0 * as;
+    } else if constexpr (M == 5) {
+        L[0] = btz * vy - bty * vz;
+        L[1] = R(0);
+        L[2] = -btz;
+        L[3] = bty;
+        L[4] = btz * sgnBx * sqrtrho;
+        L[5] = -bty * sgnBx * sqrtrho;
+        L[6] = R(0);
+    } else {
+        L[0] = af * (gam1 * vv2 - lf * vx) + as * ls * (bty * vy + btz * vz) * sgnBx;
+        L[1] = af * (gam0 * vx + lf);
+        L[2] = gam0 * af * vy - as * ls * bty * sgnBx;
+        L[3] = gam0 * af * vz - as * ls * btz * sgnBx;
+        L[4] = gam0 * af * By + cs * as * bty * sqrtrho;
+        L[5] = gam0 * af * Bz + cs * as * btz * sqrtrho;
+        L[6] = -gam0 * af;
+    }
+    // normalisation, src/ssprk.cpp:718-726
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+        if constexpr (M == 1 || M == 5) L[c] = R(0.5) * L[c];
+        else if constexpr (M == 3) L[c] = -gam0 * L[c] / cs2;
+        else L[c] = R(0.5) * L[c] / cs2;
+    }
+    // sign of the transverse field, src/ssprk.cpp:796-806
+    if constexpr (M == 2 || M == 4) {
+        if (e.slow_flip) {
+#pragma unroll
+            for (int c = 0; c < 7; ++c) L[c] *= e.sgnBt;
+        }
+    } else if constexpr (M == 0 || M == 6) {
+        if (!e.slow_flip) {
+#pragma unroll
+            for (int c = 0; c < 7; ++c) L[c] *= e.sgnBt;
+        }
+    }
+}
+
+// Column m of R after the sign flip (src/ssprk.cpp:729-787, 808-818).
+template <int M, typename R>
+__device__ __forceinline__ void r_col(R gam, const Eig<R>& e, R* Rc) {
+    const R gam2 = (gam - R(2)) / (gam - R(1));
+    const R af = e.af, as = e.as, lf = e.lf, ls = e.ls, cs = e.cs, cs2 = e.cs2;
+    const R vx = e.vx, vy = e.vy, vz = e.vz, vv2 = e.vv2;
+    const R bty = e.bty, btz = e.btz, sgnBx = e.sgnBx, isr = e.invsqrtrho;
+    if constexpr (M == 0) {
+        Rc[0] = af;
+        Rc[1] = af * (vx - lf);
+        Rc[2] = af * vy + as * ls * bty * sgnBx;
+        Rc[3] = af * vz + as * ls * btz * sgnBx;
+        Rc[4] = cs * as * bty * isr;
+        Rc[5] = cs * as * btz * isr;
+        Rc[6] = af * (lf * lf - lf * vx + R(0.5) * vv2 - gam2 * cs2) + as * ls * (bty * vy + btz * vz) * sgnBx;
+    } else if constexpr (M == 1) {
+        Rc[0] = R(0);
+        Rc[1] = R(0);
+        Rc[2] = -btz;
+        Rc[3] = bty;
+        Rc[4] = -btz * sgnBx * isr;
+        Rc[5] = bty * sgnBx * isr;
+        Rc[6] = bty * vz - btz * vy;
+    } else if constexpr (M == 2) {
+        Rc[0] = as;
+        Rc[1] = as * (vx - ls);
+        Rc[2] = as * vy - af * lf * bty * sgnBx;
+        Rc[3] = as * vz - af * lf * btz * sgnBx;
+        Rc[4] = -cs * af * bty * isr;
+        Rc[5] = -cs * af * btz * isr;
+        Rc[6] = as * (ls * ls - ls * vx + R(0.5) * vv2 - gam2 * cs2) - af * lf * (bty * vy + btz * vz) * sgnBx;
+    } else if constexpr (M == 3) {
+        Rc[0] = R(1);
+        Rc[1] = vx;
+        Rc[2] = vy;
+        Rc[3] = vz;
+        Rc[4] = R(0);
+        Rc[5] = R(0);
+        Rc[6] = R(0.5) * vv2;
+    } else if constexpr (M == 4) {
+        Rc[0] = as;
+        Rc[1] = as * (vx + ls);
+        Rc[2] = as * vy + af * lf * bty * sgnBx;
+        Rc[3] = as * vz + af * lf * btz * sgnBx;
+        Rc[4] = -cs * af * bty * isr;
+        Rc[5] = -cs * af * btz * isr;
+        Rc[6] = as * (ls * ls + ls * vx + R(0.5) * vv2 - gam2 * cs2) + af * lf * (bty * vy + btz * vz) * sgnBx;
+    } else if constexpr (M == 5) {
+        Rc[0] = R(0);
+        Rc[1] = R(0);
+        Rc[2] = -btz;
+        Rc[3] = bty;
+        Rc[4] = btz * sgnBx * isr;
+        Rc[5] = -bty * sgnBx * isr;
+        Rc[6] = bty * vz - btz * vy;
+    } else {
+        Rc[0] = af;
+        Rc[1] = af * (vx + lf);
+        Rc[2] = af * vy - as * ls * bty * sgnBx;
+        Rc[3] = af * vz - as * ls * btz * sgnBx;
+        Rc[4] = cs * as * bty * isr;
+        Rc[5] = cs * as * btz * isr;
+        Rc[6] = af * (lf * lf + lf * vx + R(0.5) * vv2 - gam2 * cs2) - as * ls * (bty * vy + btz * vz) * sgnBx;
+    }
+    if constexpr (M == 2 || M == 4) {
+        if (e.slow_flip) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) Rc[r] *= e.sgnBt;
+        }
+    } else if constexpr (M == 0 || M == 6) {
+        if (!e.slow_flip) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) Rc[r] *= e.sgnBt;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ WENO5
+// One side of the Jiang-Shu WENO5 correction (src/ssprk.cpp:891-912):
+// phi_N(a,b,c,d) = w0 (a-2b+c)/3 + (w2-1/2)(b-2c+d)/6 with
+// IS0 = 13(a-b)^2 + 3(a-3b)^2, IS1 = 13(b-c)^2 + 3(b+c)^2,
+// IS2 = 13(c-d)^2 + 3(3c-d)^2, alpha = (1,6,3)/(eps+IS)^2, eps = 1e-10.
+template <typename R>
+__device__ __forceinline__ R weno_side(R a, R b, R c, R d, R* w0_out = nullptr, R* w2_out = nullptr) {
+    const R eps = R(1.0e-10);
+    const R IS0 = R(13) * (a - b) * (a - b) + R(3) * (a - R(3) * b) * (a - R(3) * b);
+    const R IS1 = R(13) * (b - c) * (b - c) + R(3) * (b + c) * (b + c);
+    const R IS2 = R(13) * (c - d) * (c - d) + R(3) * (R(3) * c - d) * (R(3) * c - d);
+#if HOWMHD_EXACT
+    const R alpha0 = R(1) / ((eps + IS0) * (eps + IS0));
+    const R alpha1 = R(6) / ((eps + IS1) * (eps + IS1));
+    const R alpha2 = R(3) / ((eps + IS2) * (eps + IS2));
+    const R omega0 = alpha0 / (alpha0 + alpha1 + alpha2);
+    const R omega2 = alpha2 / (alpha0 + alpha1 + alpha2);
+    if (w0_out) { *w0_out = omega0; *w2_out = omega2; }
+    return omega0 * (a - R(2) * b + c) / R(3) + (omega2 - R(0.5)) * (b - R(2) * c + d) / R(6);
+#else
+    // alpha_k = d_k / t_k with t_k = (eps+IS_k)^2: the normalised weights are
+    // w0 = t1 t2 / S, w2 = 3 t0 t1 / S with S = t1 t2 + 6 t0 t2 + 3 t0 t1,
+    // one division instead of five.  t_k >= 1e-20 and products stay far
+    // inside the double range; the float form keeps the per-alpha
+    // reciprocals, whose products could leave the float range.
+    const R t0 = (eps + IS0) * (eps + IS0);
+    const R t1 = (eps + IS1) * (eps + IS1);
+    const R t2 = (eps + IS2) * (eps + IS2);
+    R omega0, omega2;
+    if constexpr (sizeof(R) == 8) {
+        const R t01 = t0 * t1;
+        const R t12 = t1 * t2;
+        const R inv = R(1) / (t12 + R(6) * t0 * t2 + R(3) * t01);
+        omega0 = t12 * inv;
+        omega2 = R(3) * t01 * inv;
+    } else {
+        const R alpha0 = R(1) / t0;
+        const R alpha1 = R(6) / t1;
+        const R alpha2 = R(3) / t2;
+        const R inv = R(1) / (alpha0 + alpha1 + alpha2);
+        omega0 = alpha0 * inv;
+        omega2 = alpha2 * inv;
+    }
+    if (w0_out) { *w0_out = omega0; *w2_out = omega2; }
+    return omega0 * (a - R(2) * b + c) * R(1.0 / 3.0) + (omega2 - R(0.5)) * (b - R(2) * c + d) * R(1.0 / 6.0);
+#endif
+}
+
+// Characteristic flux of one wave from its 6 projected values
+// (src/ssprk.cpp:883-937): first - phi(plus side) + phi(minus side).
+template <typename R>
+__device__ __forceinline__ R weno_wave(const R* Fsk, const R* qsk, R amx) {
+    R dF[5], dq[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        dF[k] = Fsk[k + 1] - Fsk[k];
+        dq[k] = qsk[k + 1] - qsk[k];
+    }
+    const R first = div_c<R>(-Fsk[1] + R(7) * Fsk[2] + R(7) * Fsk[3] - Fsk[4], R(12), R(1.0 / 12.0));
+    const R second = weno_side<R>(R(0.5) * (dF[0] + amx * dq[0]), R(0.5) * (dF[1] + amx * dq[1]),
+                                  R(0.5) * (dF[2] + amx * dq[2]), R(0.5) * (dF[3] + amx * dq[3]));
+    const R third = weno_side<R>(R(0.5) * (dF[4] - amx * dq[4]), R(0.5) * (dF[3] - amx * dq[3]),
+                                 R(0.5) * (dF[2] - amx * dq[2]), R(0.5) * (dF[1] - amx * dq[1]));
+    return first - second + third;
+}
+
+}  // namespace howmhd

@@ -1,0 +1,171 @@
+// how_mhd — drop-in driver for `./bin/how_mhd_mpi input.in` at one rank.
+//
+// Restates src/main.cpp:25-248: read the input (argv[1], or stdin), finalise
+// the parameters, initialise the problem on the host, hand q(0) to the GPU
+// library, build the face field, write output 0, then for each output
+// interval run CFL-limited hydro steps (dt clipped to the next output time,
+// abort on dt <= 0 or a non-finite time) and write the output.  Added: `nstep`
+// > 0 stops after that many hydro steps, and a closing [timing] line reports
+// zone-cycles/s of the step loop.
+#include "howmhd_gpu.h"
+#include "howmhd_host.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+void print_banner() {
+    std::cout << "\n"
+              << "  HOW-MHD  |  WENO5  |  SSPRK(5,4)  |  4th-order CT  |  MI355X (gfx950)\n"
+              << "  High-order WENO magnetohydrodynamics, single-GPU HIP time step\n"
+              << "  Method: Seo & Ryu (2023), ApJ 953, 39\n\n";
+}
+
+// print_log, src/main.cpp:275-341 (one rank)
+void print_log(const howmhd_run& par) {
+    std::cout << "\n";
+    std::cout << "===== LOG rank 0 =====\n";
+    std::cout << "MPI ranks          = 1\n";
+    std::cout << "MPI layout         = 1 x 1 x 1\n";
+    std::cout << "rank coords        = (0, 0, 0)\n";
+    std::cout << "global offset      = 0 0 0\n";
+    std::cout << "global nx ny nz    = " << par.gnx << " " << par.gny << " " << par.gnz << "\n";
+    std::cout << "local  nx ny nz    = " << par.nx << " " << par.ny << " " << par.nz << "\n";
+    std::cout << "nt                 = " << par.nt << "\n";
+    std::cout << "xsize ysize zsize  = " << par.xsize << " " << par.ysize << " " << par.zsize << "\n";
+    std::cout << "tend               = " << par.tend << "\n";
+    std::cout << "dx dy dz           = " << par.dx << " " << par.dy << " " << par.dz << "\n";
+    std::cout << "boundary           = " << par.x1bc << " " << par.x2bc << " " << par.x3bc << "\n";
+    std::cout << "output_format      = " << par.output_format << "\n";
+    std::cout << "precision          = " << par.precision << "\n";
+    std::cout << "indicator          = " << par.indicator << "\n";
+    std::cout << "============================\n";
+}
+
+[[noreturn]] void die(const std::string& msg) {
+    std::cerr << msg << "\n";
+    std::abort();
+}
+
+void check(int rc, howmhd_gpu_ctx* ctx, const char* what) {
+    if (rc != HOWMHD_OK) die(std::string("Error: ") + what + ": " + howmhd_gpu_last_error(ctx));
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+    std::cout.setf(std::ios::unitbuf);
+    std::cerr.setf(std::ios::unitbuf);
+
+    std::string text;
+    if (argc > 1) {
+        std::ifstream f(argv[1]);
+        if (!f) die(std::string("Error: cannot open input file: ") + argv[1]);
+        std::stringstream ss;
+        ss << f.rdbuf();
+        text = ss.str();
+    } else {
+        std::stringstream ss;
+        ss << std::cin.rdbuf();
+        text = ss.str();
+    }
+
+    char err[4096] = {0};
+    howmhd_run par;
+    if (howmhd_host_parse(text.c_str(), &par, err, sizeof(err)) != 0) die(err);
+    if (par.px > 1 || par.py > 1 || par.pz > 1)
+        std::cerr << "Warning: px/py/pz were provided, but this build runs one rank on one GPU. "
+                  << "Ignoring MPI layout and using one rank.\n";
+
+    print_banner();
+    print_log(par);
+
+    howmhd_params gp;
+    if (howmhd_host_gpu_params(&par, &gp, err, sizeof(err)) != 0) die(err);
+
+    const size_t ncell = (size_t)(par.nx + 6) * (par.ny + 6) * (par.nz + 6);
+    std::vector<double> q0(8 * ncell, 0.0);
+    if (howmhd_host_init_problem(&par, q0.data(), err, sizeof(err)) != 0) die(err);
+    // reject an unknown output format before any step runs (src/output.cpp:565-569)
+    {
+        const std::string f = par.output_format;
+        if (f != "dat" && f != "vtk" && f != "vts" && f != "none")
+            die("Error: unknown output_format = " + f + "\nAvailable options: dat, vtk, none");
+    }
+
+    howmhd_gpu_ctx* ctx = nullptr;
+    if (howmhd_gpu_create(&gp, &ctx) != 0) die(std::string("Error: ") + howmhd_gpu_last_error(nullptr));
+    check(howmhd_gpu_upload_q0(ctx, q0.data()), ctx, "upload_q0");
+    check(howmhd_gpu_init_faces(ctx), ctx, "init_faces");
+
+    double time = 0.0;
+    const double dtout = par.tend / static_cast<double>(par.nt);
+    std::cout << "dtout = " << dtout << std::endl;
+
+    auto write_output = [&](int step, double t) {
+        if (std::string(par.output_format) == "none") return;
+        check(howmhd_gpu_download_q0(ctx, q0.data()), ctx, "download_q0");
+        if (howmhd_host_output(&par, q0.data(), step, t, "", err, sizeof(err)) != 0) die(err);
+    };
+
+    write_output(0, time);
+    std::cout << "output step = 0 time = " << time << std::endl;
+
+    long total_steps = 0;
+    double loop_seconds = 0.0;
+    bool stop = false;
+    for (int out_step = 1; out_step <= par.nt && !stop; ++out_step) {
+        const double target_time = dtout * static_cast<double>(out_step);
+        int hydro_step = 0;
+        double dt = 0.0;
+        const auto t0 = std::chrono::steady_clock::now();
+        while (time < target_time) {
+            if (par.nstep > 0 && total_steps >= par.nstep) { stop = true; break; }
+            check(howmhd_gpu_tstep(ctx, &dt), ctx, "tstep");
+            if (time + dt > target_time) dt = target_time - time;
+            if (dt <= 0.0) {
+                std::cerr << "Error: non-positive dt detected.\n";
+                std::cerr << "time        = " << time << "\n";
+                std::cerr << "target_time = " << target_time << "\n";
+                std::cerr << "dt          = " << dt << "\n";
+                break;
+            }
+            if (!par.quiet)
+                std::cout << "[advance begin] out_step = " << out_step << " hydro_step = " << hydro_step + 1
+                          << " time = " << time << " dt = " << dt << " target_time = " << target_time << std::endl;
+            check(howmhd_gpu_ssprk(ctx, dt), ctx, "ssprk");
+            time += dt;
+            hydro_step++;
+            total_steps++;
+            if (!par.quiet)
+                std::cout << "[advance done ] out_step = " << out_step << " hydro_step = " << hydro_step
+                          << " time = " << time << " dt = " << dt << std::endl;
+            if (!std::isfinite(time)) {
+                std::cerr << "Error: time became non-finite.\n";
+                break;
+            }
+            if (hydro_step % 50 == 0)
+                std::cout << "hydro_step = " << hydro_step << " time = " << time << " dt = " << dt
+                          << " target_time = " << target_time << std::endl;
+        }
+        check(howmhd_gpu_synchronize(ctx), ctx, "synchronize");
+        loop_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        write_output(out_step, time);
+        std::cout << "output step = " << out_step << " time = " << time << " last_dt = " << dt
+                  << " hydro_steps = " << hydro_step << std::endl;
+    }
+    const double zones = (double)par.gnx * par.gny * par.gnz;
+    std::cout << "[timing] hydro_steps = " << total_steps << " loop_seconds = " << loop_seconds
+              << " zone_cycles_per_s = " << (loop_seconds > 0 ? zones * total_steps / loop_seconds : 0.0)
+              << std::endl;
+    howmhd_gpu_destroy(ctx);
+    return 0;
+}

@@ -1,0 +1,127 @@
+/*
+ * howmhd_gpu.h — C-ABI of the MI355X (gfx950) HOW-MHD time-step library.
+ *
+ * The library owns the device state of one single-GPU run: the conserved
+ * variables q = (rho, Mx, My, Mz, Bx, By, Bz, E) of every SSPRK(5,4) stage and
+ * the face-centred magnetic field (bxb, byb, bzb) used by the 4th-order
+ * constrained transport.  Host arrays cross this boundary in the reference
+ * layout: q(m,i,j,k) with k fastest, padded by 3 ghost cells per side, i.e.
+ * 8 x (nx+6) x (ny+6) x (nz+6) doubles (src/main.cpp:114-120 without the stage
+ * index).  On the device only active cells are stored; ghost values are the
+ * periodic or open images of active cells, exactly as src/bound.cpp and the
+ * serial paths of src/fluxct.cpp leave them.
+ *
+ * Conventions: every call returns 0 on success and a non-zero HOWMHD_E* code
+ * on failure; howmhd_gpu_last_error() then holds a message.  Calls are ordered
+ * on the context's stream and synchronise only where a host value is produced
+ * (tstep, downloads, diag).  A context belongs to one host thread.
+ */
+#ifndef HOWMHD_GPU_H
+#define HOWMHD_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HOWMHD_OK = 0,
+    HOWMHD_EINVAL = 1,   /* bad argument or unsupported configuration   */
+    HOWMHD_EHIP = 2,     /* a HIP runtime call failed                   */
+    HOWMHD_ENOMEM = 3,   /* device allocation failed                    */
+    HOWMHD_ESTATE = 4    /* call out of order (e.g. ssprk before faces) */
+};
+
+enum { HOWMHD_BC_PERIODIC = 0, HOWMHD_BC_OPEN = 1 };
+
+enum {
+    HOWMHD_PREC_FP64 = 0,  /* plain method, all arithmetic in FP64                */
+    HOWMHD_PREC_FP32 = 1   /* sweeps and CT EMFs in FP32; stage sums, E->p, floors
+                              and the face-B accumulation stay in FP64             */
+};
+
+enum {
+    HOWMHD_INDICATOR_OFF = 0, /* characteristic WENO5 at every interface            */
+    HOWMHD_INDICATOR_MLP = 1  /* learned smoothness indicator routes interfaces it
+                                 calls smooth to linear upwind-5 reconstruction   */
+};
+
+/* Run parameters.  Mirrors the fields of the reference `Parameters`
+ * (src/parameters.hpp:10-76) that the time step reads. */
+typedef struct howmhd_params {
+    int32_t nx, ny, nz;      /* active cells (one rank: gnx, gny, gnz)           */
+    int32_t bc[3];           /* x1bc, x2bc, x3bc: HOWMHD_BC_*                    */
+    double dx, dy, dz;       /* xsize/gnx ... (src/parameters.cpp:149-151)        */
+    double gam, cour, rhomin, pgmin;
+    int32_t precision;       /* HOWMHD_PREC_*                                     */
+    int32_t indicator;       /* HOWMHD_INDICATOR_*                                */
+} howmhd_params;
+
+/* Diagnostics of stage-0 state. */
+typedef struct howmhd_diag {
+    double sum_cons[8];      /* sum over active cells of each conserved variable */
+    double max_divb_ct;      /* max |sum_d D4_d(b_d)/Delta_d| over active cells   */
+    double max_divb_cell;    /* max |2nd-order cell-centred divB| (output.cpp:314-347) */
+    double min_rho, min_p;
+    double flagged_fraction; /* indicator: flagged interfaces / all, last step   */
+} howmhd_diag;
+
+typedef struct howmhd_gpu_ctx howmhd_gpu_ctx;
+
+/* Allocates q (5 stage buffers), faces (5 stage buffers) and the sweep / CT
+ * scratch on the current HIP device.  Replaces the allocation of q
+ * (src/main.cpp:114-120) and allocate_bfield_ct (src/fluxct.cpp:27-35).
+ * Axes must have n == 1 (degenerate) or n >= 3. */
+int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out);
+
+/* Frees everything.  Replaces deallocate_bfield_ct (src/fluxct.cpp:37-42). */
+int howmhd_gpu_destroy(howmhd_gpu_ctx* ctx);
+
+/* Copies stage-0 q from the host (reference layout, ghosts ignored).  The
+ * reference's problem initialisers write q(0,...) in place
+ * (src/problem.cpp:39-59); this is the device hand-over of that array. */
+int howmhd_gpu_upload_q0(howmhd_gpu_ctx* ctx, const double* q0);
+
+/* bound(q,0) + initialize_bfield_from_q + bound_bfield(0)
+ * (src/main.cpp:127-130, src/fluxct.cpp:65-107). */
+int howmhd_gpu_init_faces(howmhd_gpu_ctx* ctx);
+
+/* Stage-0 faces from the host, 3 x (nx+6) x (ny+6) x (nz+6) (bxb, byb, bzb),
+ * for restart or tests; ghosts ignored. */
+int howmhd_gpu_upload_faces(howmhd_gpu_ctx* ctx, const double* b);
+
+/* CFL step: dt = cour / sum_d max(vmax_d,1e-300)/Delta_d over active cells of
+ * q(0) (src/tstep.cpp:11-136).  Synchronises. */
+int howmhd_gpu_tstep(howmhd_gpu_ctx* ctx, double* dt);
+
+/* One SSPRK(5,4) step of size dt (src/ssprk.cpp:1277-1329). */
+int howmhd_gpu_ssprk(howmhd_gpu_ctx* ctx, double dt);
+
+/* q(0) to the host in the reference layout, ghosts filled with the
+ * periodic/open images (what src/bound.cpp leaves there).  Synchronises. */
+int howmhd_gpu_download_q0(howmhd_gpu_ctx* ctx, double* q0);
+
+/* Stage-0 faces to the host, 3 x (nx+6) x (ny+6) x (nz+6): bxb, byb, bzb
+ * (src/fluxct.cpp:10-12 at stage 0).  Synchronises. */
+int howmhd_gpu_download_faces(howmhd_gpu_ctx* ctx, double* b);
+
+/* Diagnostics of the stage-0 state.  Synchronises. */
+int howmhd_gpu_diag(howmhd_gpu_ctx* ctx, howmhd_diag* d);
+
+/* Waits for all work queued on the context's stream. */
+int howmhd_gpu_synchronize(howmhd_gpu_ctx* ctx);
+
+/* Message of the last failing call on ctx (or of the last failing create
+ * when ctx is NULL).  Never NULL. */
+const char* howmhd_gpu_last_error(const howmhd_gpu_ctx* ctx);
+
+/* Build identification: "fp-exact" for the library built with FP contraction
+ * off (bitwise twin of the CPU restatement), "fp-fast" for the product build. */
+const char* howmhd_gpu_build_kind(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HOWMHD_GPU_H */

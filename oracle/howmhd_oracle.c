@@ -1,0 +1,1050 @@
+/*
+ * howmhd_oracle.c — CPU restatement of the HOW-MHD time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker and the CPU baseline.  Nothing
+ * in the product links or loads this file.
+ *
+ * Every function cites the reference lines it follows.  Arithmetic is written
+ * in the reference's evaluation order and the library is built with
+ * -ffp-contract=off, so each value is rounded as the reference rounds it
+ * (x86-64 without FMA, as the reference Makefile builds it).  The one
+ * deliberate choice: the reference accumulates dF with atomic_add
+ * (src/ssprk.cpp:941-943), whose order is not fixed under OpenMP; here the
+ * waves are added in the order m = 0..6, which is the Serial-backend order.
+ */
+#include "howmhd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NVAR 8
+#define NSTAGE 6
+
+struct howmhd_ref_ctx {
+    howmhd_params p;
+    int n6x, n6y, n6z;
+    size_t ncell;          /* (nx+6)(ny+6)(nz+6) */
+    double* q;             /* NSTAGE x NVAR x ncell */
+    double* bx;            /* NSTAGE x ncell        */
+    double* by;
+    double* bz;
+    double* ct[12];        /* fsy fsz gsx gsz hsx hsy Ox2 Oy2 Oz2 Ox Oy Oz */
+    double dt, dtdx, dtdy, dtdz;
+};
+
+enum { FSY, FSZ, GSX, GSZ, HSX, HSY, OX2, OY2, OZ2, OX, OY, OZ };
+
+/* ------------------------------------------------------------------ indexing */
+#define CIDX(c, i, j, k) ((((size_t)(i)) * (size_t)(c)->n6y + (size_t)(j)) * (size_t)(c)->n6z + (size_t)(k))
+#define Q(c, s, m, i, j, k) ((c)->q[((size_t)(s) * NVAR + (size_t)(m)) * (c)->ncell + CIDX(c, i, j, k)])
+#define BX(c, s, i, j, k) ((c)->bx[(size_t)(s) * (c)->ncell + CIDX(c, i, j, k)])
+#define BY(c, s, i, j, k) ((c)->by[(size_t)(s) * (c)->ncell + CIDX(c, i, j, k)])
+#define BZ(c, s, i, j, k) ((c)->bz[(size_t)(s) * (c)->ncell + CIDX(c, i, j, k)])
+#define A3(c, a, i, j, k) ((a)[CIDX(c, i, j, k)])
+
+static double sign1(double x) { return (x >= 0.0) ? 1.0 : -1.0; }  /* ssprk.cpp:54-57 */
+
+/* RK coefficients, src/ssprk.cpp:66-94 */
+typedef struct { double k1, k2, k3, k4, k5; } rkc;
+static rkc rk_coeff(int iw) {
+    rkc c = {0.0, 0.0, 0.0, 0.0, 0.0};
+    switch (iw) {
+    case 1: c.k1 = 1.0; c.k2 = 0.39175222700392; c.k3 = 0.0; break;
+    case 2: c.k1 = 0.44437049406734; c.k2 = 0.36841059262959; c.k3 = 0.55562950593266; break;
+    case 3: c.k1 = 0.62010185138540; c.k2 = 0.25189177424738; c.k3 = 0.37989814861460; break;
+    case 4: c.k1 = 0.17807995410773; c.k2 = 0.54497475021237; c.k3 = 0.82192004589227; break;
+    case 5:
+        c.k1 = -2.081261929715610e-02; c.k2 = 0.22600748319395; c.k3 = 5.03580947213895e-1;
+        c.k4 = 0.51723167208978; c.k5 = -6.518979800418380e-12;
+        break;
+    }
+    return c;
+}
+
+/* ================================================================ bound ==== */
+/* src/bound.cpp:19-107 and 295-317 (serial path).  The six assignments of each
+ * transverse point run in the reference's order, so that degenerate axes
+ * (n=1) see the same sequence of reads. */
+static void bound_q(howmhd_ref_ctx* c, int s) {
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const int n6x = c->n6x, n6y = c->n6y, n6z = c->n6z;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int m = 0; m < NVAR; ++m)
+        for (int j = 0; j < n6y; ++j)
+            for (int k = 0; k < n6z; ++k) {
+                if (c->p.bc[0] == HOWMHD_BC_PERIODIC) {
+                    Q(c, s, m, 0, j, k) = Q(c, s, m, nx, j, k);
+                    Q(c, s, m, 1, j, k) = Q(c, s, m, nx + 1, j, k);
+                    Q(c, s, m, 2, j, k) = Q(c, s, m, nx + 2, j, k);
+                    Q(c, s, m, nx + 3, j, k) = Q(c, s, m, 3, j, k);
+                    Q(c, s, m, nx + 4, j, k) = Q(c, s, m, 4, j, k);
+                    Q(c, s, m, nx + 5, j, k) = Q(c, s, m, 5, j, k);
+                } else {
+                    Q(c, s, m, 0, j, k) = Q(c, s, m, 3, j, k);
+                    Q(c, s, m, 1, j, k) = Q(c, s, m, 3, j, k);
+                    Q(c, s, m, 2, j, k) = Q(c, s, m, 3, j, k);
+                    Q(c, s, m, nx + 3, j, k) = Q(c, s, m, nx + 2, j, k);
+                    Q(c, s, m, nx + 4, j, k) = Q(c, s, m, nx + 2, j, k);
+                    Q(c, s, m, nx + 5, j, k) = Q(c, s, m, nx + 2, j, k);
+                }
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int m = 0; m < NVAR; ++m)
+        for (int i = 0; i < n6x; ++i)
+            for (int k = 0; k < n6z; ++k) {
+                if (c->p.bc[1] == HOWMHD_BC_PERIODIC) {
+                    Q(c, s, m, i, 0, k) = Q(c, s, m, i, ny, k);
+                    Q(c, s, m, i, 1, k) = Q(c, s, m, i, ny + 1, k);
+                    Q(c, s, m, i, 2, k) = Q(c, s, m, i, ny + 2, k);
+                    Q(c, s, m, i, ny + 3, k) = Q(c, s, m, i, 3, k);
+                    Q(c, s, m, i, ny + 4, k) = Q(c, s, m, i, 4, k);
+                    Q(c, s, m, i, ny + 5, k) = Q(c, s, m, i, 5, k);
+                } else {
+                    Q(c, s, m, i, 0, k) = Q(c, s, m, i, 3, k);
+                    Q(c, s, m, i, 1, k) = Q(c, s, m, i, 3, k);
+                    Q(c, s, m, i, 2, k) = Q(c, s, m, i, 3, k);
+                    Q(c, s, m, i, ny + 3, k) = Q(c, s, m, i, ny + 2, k);
+                    Q(c, s, m, i, ny + 4, k) = Q(c, s, m, i, ny + 2, k);
+                    Q(c, s, m, i, ny + 5, k) = Q(c, s, m, i, ny + 2, k);
+                }
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int m = 0; m < NVAR; ++m)
+        for (int i = 0; i < n6x; ++i)
+            for (int j = 0; j < n6y; ++j) {
+                if (c->p.bc[2] == HOWMHD_BC_PERIODIC) {
+                    Q(c, s, m, i, j, 0) = Q(c, s, m, i, j, nz);
+                    Q(c, s, m, i, j, 1) = Q(c, s, m, i, j, nz + 1);
+                    Q(c, s, m, i, j, 2) = Q(c, s, m, i, j, nz + 2);
+                    Q(c, s, m, i, j, nz + 3) = Q(c, s, m, i, j, 3);
+                    Q(c, s, m, i, j, nz + 4) = Q(c, s, m, i, j, 4);
+                    Q(c, s, m, i, j, nz + 5) = Q(c, s, m, i, j, 5);
+                } else {
+                    Q(c, s, m, i, j, 0) = Q(c, s, m, i, j, 3);
+                    Q(c, s, m, i, j, 1) = Q(c, s, m, i, j, 3);
+                    Q(c, s, m, i, j, 2) = Q(c, s, m, i, j, 3);
+                    Q(c, s, m, i, j, nz + 3) = Q(c, s, m, i, j, nz + 2);
+                    Q(c, s, m, i, j, nz + 4) = Q(c, s, m, i, j, nz + 2);
+                    Q(c, s, m, i, j, nz + 5) = Q(c, s, m, i, j, nz + 2);
+                }
+            }
+}
+
+/* One 3-D array (a face component or a CT array): src/fluxct.cpp:141-205 and
+ * 510-552 apply the same x, then y, then z copy to each array. */
+static void bound_a3(howmhd_ref_ctx* c, double* a) {
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const int n6x = c->n6x, n6y = c->n6y, n6z = c->n6z;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n6y; ++j)
+        for (int k = 0; k < n6z; ++k) {
+            if (c->p.bc[0] == HOWMHD_BC_PERIODIC) {
+                A3(c, a, 0, j, k) = A3(c, a, nx, j, k);
+                A3(c, a, 1, j, k) = A3(c, a, nx + 1, j, k);
+                A3(c, a, 2, j, k) = A3(c, a, nx + 2, j, k);
+                A3(c, a, nx + 3, j, k) = A3(c, a, 3, j, k);
+                A3(c, a, nx + 4, j, k) = A3(c, a, 4, j, k);
+                A3(c, a, nx + 5, j, k) = A3(c, a, 5, j, k);
+            } else {
+                A3(c, a, 0, j, k) = A3(c, a, 3, j, k);
+                A3(c, a, 1, j, k) = A3(c, a, 3, j, k);
+                A3(c, a, 2, j, k) = A3(c, a, 3, j, k);
+                A3(c, a, nx + 3, j, k) = A3(c, a, nx + 2, j, k);
+                A3(c, a, nx + 4, j, k) = A3(c, a, nx + 2, j, k);
+                A3(c, a, nx + 5, j, k) = A3(c, a, nx + 2, j, k);
+            }
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n6x; ++i)
+        for (int k = 0; k < n6z; ++k) {
+            if (c->p.bc[1] == HOWMHD_BC_PERIODIC) {
+                A3(c, a, i, 0, k) = A3(c, a, i, ny, k);
+                A3(c, a, i, 1, k) = A3(c, a, i, ny + 1, k);
+                A3(c, a, i, 2, k) = A3(c, a, i, ny + 2, k);
+                A3(c, a, i, ny + 3, k) = A3(c, a, i, 3, k);
+                A3(c, a, i, ny + 4, k) = A3(c, a, i, 4, k);
+                A3(c, a, i, ny + 5, k) = A3(c, a, i, 5, k);
+            } else {
+                A3(c, a, i, 0, k) = A3(c, a, i, 3, k);
+                A3(c, a, i, 1, k) = A3(c, a, i, 3, k);
+                A3(c, a, i, 2, k) = A3(c, a, i, 3, k);
+                A3(c, a, i, ny + 3, k) = A3(c, a, i, ny + 2, k);
+                A3(c, a, i, ny + 4, k) = A3(c, a, i, ny + 2, k);
+                A3(c, a, i, ny + 5, k) = A3(c, a, i, ny + 2, k);
+            }
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n6x; ++i)
+        for (int j = 0; j < n6y; ++j) {
+            if (c->p.bc[2] == HOWMHD_BC_PERIODIC) {
+                A3(c, a, i, j, 0) = A3(c, a, i, j, nz);
+                A3(c, a, i, j, 1) = A3(c, a, i, j, nz + 1);
+                A3(c, a, i, j, 2) = A3(c, a, i, j, nz + 2);
+                A3(c, a, i, j, nz + 3) = A3(c, a, i, j, 3);
+                A3(c, a, i, j, nz + 4) = A3(c, a, i, j, 4);
+                A3(c, a, i, j, nz + 5) = A3(c, a, i, j, 5);
+            } else {
+                A3(c, a, i, j, 0) = A3(c, a, i, j, 3);
+                A3(c, a, i, j, 1) = A3(c, a, i, j, 3);
+                A3(c, a, i, j, 2) = A3(c, a, i, j, 3);
+                A3(c, a, i, j, nz + 3) = A3(c, a, i, j, nz + 2);
+                A3(c, a, i, j, nz + 4) = A3(c, a, i, j, nz + 2);
+                A3(c, a, i, j, nz + 5) = A3(c, a, i, j, nz + 2);
+            }
+        }
+}
+
+/* bound_bfield for one stage: src/fluxct.cpp:141-205 fills bxb, byb and bzb
+ * inside one kernel per axis; axis by axis the result equals filling each
+ * array on its own, since the arrays do not read each other. */
+static void bound_faces(howmhd_ref_ctx* c, int s) {
+    bound_a3(c, c->bx + (size_t)s * c->ncell);
+    bound_a3(c, c->by + (size_t)s * c->ncell);
+    bound_a3(c, c->bz + (size_t)s * c->ncell);
+}
+
+/* ====================================================== one-line pipeline == */
+typedef struct {
+    int nn, N;
+    double *qx, *ux, *ax, *Fx, *Lx, *Rx, *dF, *bsy, *bsz;
+} line_ws;
+
+#define WQ(w, m, c) ((w)->qx[(size_t)(m) * (w)->N + (c)])
+#define WU(w, m, c) ((w)->ux[(size_t)(m) * (w)->N + (c)])
+#define WA(w, m, c) ((w)->ax[(size_t)(m) * (w)->N + (c)])
+#define WF(w, m, c) ((w)->Fx[(size_t)(m) * (w)->N + (c)])
+#define WL(w, r, col, c) ((w)->Lx[((size_t)(r) * 7 + (col)) * (w)->N + (c)])
+#define WR(w, r, col, c) ((w)->Rx[((size_t)(r) * 7 + (col)) * (w)->N + (c)])
+#define WD(w, r, i) ((w)->dF[(size_t)(r) * ((w)->nn + 1) + (i)])
+
+static int ws_alloc(line_ws* w, int nn) {
+    w->nn = nn;
+    w->N = nn + 6;
+    const size_t N = (size_t)w->N;
+    w->qx = (double*)calloc(8 * N, sizeof(double));
+    w->ux = (double*)calloc(9 * N, sizeof(double));
+    w->ax = (double*)calloc(7 * N, sizeof(double));
+    w->Fx = (double*)calloc(7 * N, sizeof(double));
+    w->Lx = (double*)calloc(49 * N, sizeof(double));
+    w->Rx = (double*)calloc(49 * N, sizeof(double));
+    w->dF = (double*)calloc(7 * (size_t)(nn + 1), sizeof(double));
+    w->bsy = (double*)calloc((size_t)(nn + 1), sizeof(double));
+    w->bsz = (double*)calloc((size_t)(nn + 1), sizeof(double));
+    return (w->qx && w->ux && w->ax && w->Fx && w->Lx && w->Rx && w->dF && w->bsy && w->bsz) ? 0 : -1;
+}
+
+static void ws_free(line_ws* w) {
+    free(w->qx); free(w->ux); free(w->ax); free(w->Fx);
+    free(w->Lx); free(w->Rx); free(w->dF); free(w->bsy); free(w->bsz);
+}
+
+/* primit_x_all, src/ssprk.cpp:458-516 */
+static void line_primit(line_ws* w, double gam, double rhomin, double pgmin) {
+    for (int ii = 0; ii < w->N; ++ii) {
+        const double DD = WQ(w, 0, ii), Mx = WQ(w, 1, ii), My = WQ(w, 2, ii), Mz = WQ(w, 3, ii);
+        const double Bx = WQ(w, 4, ii), By = WQ(w, 5, ii), Bz = WQ(w, 6, ii);
+        double EE = WQ(w, 7, ii);
+        double rho = DD;
+        const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
+        const double vv2 = vx * vx + vy * vy + vz * vz;
+        const double BB2 = Bx * Bx + By * By + Bz * Bz;
+        double pg = (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2));
+        if (rho < rhomin || pg < pgmin) {
+            rho = fmax(rhomin, rho);
+            pg = fmax(pgmin, pg);
+            EE = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
+            WQ(w, 0, ii) = rho;
+            WQ(w, 7, ii) = EE;
+        }
+        const double HH = (EE + pg) / rho;
+        WU(w, 0, ii) = rho; WU(w, 1, ii) = vx; WU(w, 2, ii) = vy; WU(w, 3, ii) = vz;
+        WU(w, 4, ii) = Bx; WU(w, 5, ii) = By; WU(w, 6, ii) = Bz; WU(w, 7, ii) = pg; WU(w, 8, ii) = HH;
+    }
+}
+
+/* eigenst_x_all centre part, src/ssprk.cpp:538-595 */
+static void line_center(line_ws* w, double gam) {
+    for (int ii = 0; ii < w->N; ++ii) {
+        const double DD = WQ(w, 0, ii), Mx = WQ(w, 1, ii), My = WQ(w, 2, ii), Mz = WQ(w, 3, ii);
+        const double EE = WQ(w, 7, ii);
+        const double rho = WU(w, 0, ii), vx = WU(w, 1, ii), vy = WU(w, 2, ii), vz = WU(w, 3, ii);
+        const double Bx = WU(w, 4, ii), By = WU(w, 5, ii), Bz = WU(w, 6, ii), pg = WU(w, 7, ii);
+        const double vv2 = vx * vx + vy * vy + vz * vz;
+        const double BB2 = Bx * Bx + By * By + Bz * Bz;
+        const double pt = pg + 0.5 * BB2;
+        const double vdB = vx * Bx + vy * By + vz * Bz;
+        const double bbn2 = BB2 / rho;
+        const double bnx2 = Bx * Bx / rho;
+        const double cs2 = fmax(0.0, gam * fabs(pg / rho));
+        const double root = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+        const double lf = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + root)));
+        const double la = sqrt(fmax(0.0, bnx2));
+        const double ls = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 - root)));
+        (void)vv2;
+        WA(w, 0, ii) = vx - lf; WA(w, 1, ii) = vx - la; WA(w, 2, ii) = vx - ls; WA(w, 3, ii) = vx;
+        WA(w, 4, ii) = vx + ls; WA(w, 5, ii) = vx + la; WA(w, 6, ii) = vx + lf;
+        WF(w, 0, ii) = DD * vx;
+        WF(w, 1, ii) = Mx * vx + pt - Bx * Bx;
+        WF(w, 2, ii) = My * vx - Bx * By;
+        WF(w, 3, ii) = Mz * vx - Bx * Bz;
+        WF(w, 4, ii) = By * vx - Bx * vy;
+        WF(w, 5, ii) = Bz * vx - Bx * vz;
+        WF(w, 6, ii) = (EE + pt) * vx - Bx * vdB;
+    }
+}
+
+/* Interface eigensystem, src/ssprk.cpp:610-818.  ul/ur: (rho,vn,vt1,vt2,Bn,
+ * Bt1,Bt2,p,H) of the two cells; L[r*7+c] (row = wave), R[r*7+c] (col = wave). */
+static void eigensystem(double gam, const double* ul, const double* ur, double* L, double* R) {
+    const double gam0 = 1.0 - gam;
+    const double gam1 = 0.5 * (gam - 1.0);
+    const double gam2 = (gam - 2.0) / (gam - 1.0);
+
+    const double rho = 0.5 * (ul[0] + ur[0]);
+    const double vx = 0.5 * (ul[1] + ur[1]);
+    const double vy = 0.5 * (ul[2] + ur[2]);
+    const double vz = 0.5 * (ul[3] + ur[3]);
+    const double Bx = 0.5 * (ul[4] + ur[4]);
+    const double By = 0.5 * (ul[5] + ur[5]);
+    const double Bz = 0.5 * (ul[6] + ur[6]);
+    const double pg = 0.5 * (ul[7] + ur[7]);
+    const double HH = 0.5 * (ul[8] + ur[8]);
+    (void)pg;
+
+    const double vv2 = vx * vx + vy * vy + vz * vz;
+    const double BB2 = Bx * Bx + By * By + Bz * Bz;
+    const double bbn2 = BB2 / rho;
+    const double bnx2 = Bx * Bx / rho;
+    const double cs2 = (gam - 1.0) * (HH - 0.5 * (vv2 + bbn2));
+    const double cs = sqrt(fmax(0.0, cs2));
+    const double root = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+    const double lf = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + root)));
+    const double la = sqrt(fmax(0.0, bnx2));
+    const double ls = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 - root)));
+
+    const double Bt2 = By * By + Bz * Bz;
+    const double sgnBx = sign1(Bx);
+    double bty, btz;
+    if (Bt2 >= 1.0e-30) {
+        bty = By / sqrt(Bt2);
+        btz = Bz / sqrt(Bt2);
+    } else {
+        bty = 1.0 / sqrt(2.0);
+        btz = 1.0 / sqrt(2.0);
+    }
+    double af, as;
+    if ((lf * lf - ls * ls) >= 1.0e-30) {
+        af = sqrt(fmax(0.0, cs2 - ls * ls)) / sqrt(lf * lf - ls * ls);
+        as = sqrt(fmax(0.0, lf * lf - cs2)) / sqrt(lf * lf - ls * ls);
+    } else {
+        af = 1.0;
+        as = 1.0;
+    }
+    const double sqrtrho = sqrt(rho);
+    const double invsqrtrho = 1.0 / sqrtrho;
+
+#define LL(r, col) L[(r) * 7 + (col)]
+#define RR(r, col) R[(r) * 7 + (col)]
+    LL(0, 0) = af * (gam1 * vv2 + lf * vx) - as * ls * (bty * vy + btz * vz) * sgnBx;
+    LL(0, 1) = af * (gam0 * vx - lf);
+    LL(0, 2) = gam0 * af * vy + as * ls * bty * sgnBx;
+    LL(0, 3) = gam0 * af * vz + as * ls * btz * sgnBx;
+    LL(0, 4) = gam0 * af * By + cs * as * bty * sqrtrho;
+    LL(0, 5) = gam0 * af * Bz + cs * as * btz * sqrtrho;
+    LL(0, 6) = -gam0 * af;
+
+    LL(1, 0) = btz * vy - bty * vz;
+    LL(1, 1) = 0.0;
+    LL(1, 2) = -btz;
+    LL(1, 3) = bty;
+    LL(1, 4) = -btz * sgnBx * sqrtrho;
+    LL(1, 5) = bty * sgnBx * sqrtrho;
+    LL(1, 6) = 0.0;
+
+    LL(2, 0) = as * (gam1 * vv2 + ls * vx) + af * lf * (bty * vy + btz * vz) * sgnBx;
+    LL(2, 1) = gam0 * as * vx - as * ls;
+    LL(2, 2) = gam0 * as * vy - af * lf * bty * sgnBx;
+    LL(2, 3) = gam0 * as * vz - af * lf * btz * sgnBx;
+    LL(2, 4) = gam0 * as * By - cs * af * bty * sqrtrho;
+    LL(2, 5) = gam0 * as * Bz - cs * af * btz * sqrtrho;
+    LL(2, 6) = -gam0 * as;
+
+    LL(3, 0) = -cs2 / gam0 - 0.5 * vv2;
+    LL(3, 1) = vx;
+    LL(3, 2) = vy;
+    LL(3, 3) = vz;
+    LL(3, 4) = By;
+    LL(3, 5) = Bz;
+    LL(3, 6) = -1.0;
+
+    LL(4, 0) = as * (gam1 * vv2 - ls * vx) - af * lf * (bty * vy + btz * vz) * sgnBx;
+    LL(4, 1) = as * (gam0 * vx + ls);
+    LL(4, 2) = gam0 * as * vy + af * lf * bty * sgnBx;
+    LL(4, 3) = gam0 * as * vz + af * lf * btz * sgnBx;
+    LL(4, 4) = gam0 * as * By - cs * af * bty * sqrtrho;
+    LL(4, 5) = gam0 * as * Bz - cs * af * btz * sqrtrho;
+    LL(4, 6) = -gam0 * as;
+
+    LL(5, 0) = btz * vy - bty * vz;
+    LL(5, 1) = 0.0;
+    LL(5, 2) = -btz;
+    LL(5, 3) = bty;
+    LL(5, 4) = btz * sgnBx * sqrtrho;
+    LL(5, 5) = -bty * sgnBx * sqrtrho;
+    LL(5, 6) = 0.0;
+
+    LL(6, 0) = af * (gam1 * vv2 - lf * vx) + as * ls * (bty * vy + btz * vz) * sgnBx;
+    LL(6, 1) = af * (gam0 * vx + lf);
+    LL(6, 2) = gam0 * af * vy - as * ls * bty * sgnBx;
+    LL(6, 3) = gam0 * af * vz - as * ls * btz * sgnBx;
+    LL(6, 4) = gam0 * af * By + cs * as * bty * sqrtrho;
+    LL(6, 5) = gam0 * af * Bz + cs * as * btz * sqrtrho;
+    LL(6, 6) = -gam0 * af;
+
+    for (int m = 0; m < 7; ++m) {
+        LL(0, m) = 0.5 * LL(0, m) / cs2;
+        LL(1, m) = 0.5 * LL(1, m);
+        LL(2, m) = 0.5 * LL(2, m) / cs2;
+        LL(3, m) = -gam0 * LL(3, m) / cs2;
+        LL(4, m) = 0.5 * LL(4, m) / cs2;
+        LL(5, m) = 0.5 * LL(5, m);
+        LL(6, m) = 0.5 * LL(6, m) / cs2;
+    }
+
+    RR(0, 0) = af;
+    RR(1, 0) = af * (vx - lf);
+    RR(2, 0) = af * vy + as * ls * bty * sgnBx;
+    RR(3, 0) = af * vz + as * ls * btz * sgnBx;
+    RR(4, 0) = cs * as * bty * invsqrtrho;
+    RR(5, 0) = cs * as * btz * invsqrtrho;
+    RR(6, 0) = af * (lf * lf - lf * vx + 0.5 * vv2 - gam2 * cs2) + as * ls * (bty * vy + btz * vz) * sgnBx;
+
+    RR(0, 1) = 0.0;
+    RR(1, 1) = 0.0;
+    RR(2, 1) = -btz;
+    RR(3, 1) = bty;
+    RR(4, 1) = -btz * sgnBx * invsqrtrho;
+    RR(5, 1) = bty * sgnBx * invsqrtrho;
+    RR(6, 1) = bty * vz - btz * vy;
+
+    RR(0, 2) = as;
+    RR(1, 2) = as * (vx - ls);
+    RR(2, 2) = as * vy - af * lf * bty * sgnBx;
+    RR(3, 2) = as * vz - af * lf * btz * sgnBx;
+    RR(4, 2) = -cs * af * bty * invsqrtrho;
+    RR(5, 2) = -cs * af * btz * invsqrtrho;
+    RR(6, 2) = as * (ls * ls - ls * vx + 0.5 * vv2 - gam2 * cs2) - af * lf * (bty * vy + btz * vz) * sgnBx;
+
+    RR(0, 3) = 1.0;
+    RR(1, 3) = vx;
+    RR(2, 3) = vy;
+    RR(3, 3) = vz;
+    RR(4, 3) = 0.0;
+    RR(5, 3) = 0.0;
+    RR(6, 3) = 0.5 * vv2;
+
+    RR(0, 4) = as;
+    RR(1, 4) = as * (vx + ls);
+    RR(2, 4) = as * vy + af * lf * bty * sgnBx;
+    RR(3, 4) = as * vz + af * lf * btz * sgnBx;
+    RR(4, 4) = -cs * af * bty * invsqrtrho;
+    RR(5, 4) = -cs * af * btz * invsqrtrho;
+    RR(6, 4) = as * (ls * ls + ls * vx + 0.5 * vv2 - gam2 * cs2) + af * lf * (bty * vy + btz * vz) * sgnBx;
+
+    RR(0, 5) = 0.0;
+    RR(1, 5) = 0.0;
+    RR(2, 5) = -btz;
+    RR(3, 5) = bty;
+    RR(4, 5) = btz * sgnBx * invsqrtrho;
+    RR(5, 5) = -bty * sgnBx * invsqrtrho;
+    RR(6, 5) = bty * vz - btz * vy;
+
+    RR(0, 6) = af;
+    RR(1, 6) = af * (vx + lf);
+    RR(2, 6) = af * vy - as * ls * bty * sgnBx;
+    RR(3, 6) = af * vz - as * ls * btz * sgnBx;
+    RR(4, 6) = cs * as * bty * invsqrtrho;
+    RR(5, 6) = cs * as * btz * invsqrtrho;
+    RR(6, 6) = af * (lf * lf + lf * vx + 0.5 * vv2 - gam2 * cs2) - as * ls * (bty * vy + btz * vz) * sgnBx;
+
+    double sgnBt;
+    if (By != 0.0) sgnBt = sign1(By);
+    else sgnBt = sign1(Bz);
+
+    if (cs >= la) {
+        for (int m = 0; m < 7; ++m) { LL(2, m) *= sgnBt; LL(4, m) *= sgnBt; }
+        for (int m = 0; m < 7; ++m) { RR(m, 2) *= sgnBt; RR(m, 4) *= sgnBt; }
+    } else {
+        for (int m = 0; m < 7; ++m) { LL(0, m) *= sgnBt; LL(6, m) *= sgnBt; }
+        for (int m = 0; m < 7; ++m) { RR(m, 0) *= sgnBt; RR(m, 6) *= sgnBt; }
+    }
+#undef LL
+#undef RR
+}
+
+/* eigenst_x_all boundary part, src/ssprk.cpp:602-820: interfaces ii = 1..nn+3 */
+static void line_boundary(line_ws* w, double gam) {
+    double ul[9], ur[9], L[49], R[49];
+    for (int ii = 1; ii < w->nn + 4; ++ii) {
+        for (int v = 0; v < 9; ++v) { ul[v] = WU(w, v, ii); ur[v] = WU(w, v, ii + 1); }
+        eigensystem(gam, ul, ur, L, R);
+        for (int r = 0; r < 7; ++r)
+            for (int col = 0; col < 7; ++col) {
+                WL(w, r, col, ii) = L[r * 7 + col];
+                WR(w, r, col, ii) = R[r * 7 + col];
+            }
+    }
+}
+
+/* Characteristic LLF-WENO5 flux of one wave, src/ssprk.cpp:883-937 */
+static double weno_wave(const double* Fsk, const double* qsk, double amx, double* wp, double* wm) {
+    const double eps = 1.0e-10;
+    double dFsk[5], dqsk[5];
+    for (int ks = 0; ks < 5; ++ks) {
+        dFsk[ks] = Fsk[ks + 1] - Fsk[ks];
+        dqsk[ks] = qsk[ks + 1] - qsk[ks];
+    }
+    const double first = (-Fsk[1] + 7.0 * Fsk[2] + 7.0 * Fsk[3] - Fsk[4]) / 12.0;
+
+    double aterm = 0.5 * (dFsk[0] + amx * dqsk[0]);
+    double bterm = 0.5 * (dFsk[1] + amx * dqsk[1]);
+    double cterm = 0.5 * (dFsk[2] + amx * dqsk[2]);
+    double dterm = 0.5 * (dFsk[3] + amx * dqsk[3]);
+    double IS0 = 13.0 * (aterm - bterm) * (aterm - bterm) + 3.0 * (aterm - 3.0 * bterm) * (aterm - 3.0 * bterm);
+    double IS1 = 13.0 * (bterm - cterm) * (bterm - cterm) + 3.0 * (bterm + cterm) * (bterm + cterm);
+    double IS2 = 13.0 * (cterm - dterm) * (cterm - dterm) + 3.0 * (3.0 * cterm - dterm) * (3.0 * cterm - dterm);
+    double alpha0 = 1.0 / ((eps + IS0) * (eps + IS0));
+    double alpha1 = 6.0 / ((eps + IS1) * (eps + IS1));
+    double alpha2 = 3.0 / ((eps + IS2) * (eps + IS2));
+    double omega0 = alpha0 / (alpha0 + alpha1 + alpha2);
+    double omega2 = alpha2 / (alpha0 + alpha1 + alpha2);
+    if (wp) { wp[0] = omega0; wp[1] = alpha1 / (alpha0 + alpha1 + alpha2); wp[2] = omega2; }
+    const double second = omega0 * (aterm - 2.0 * bterm + cterm) / 3.0
+                        + (omega2 - 0.5) * (bterm - 2.0 * cterm + dterm) / 6.0;
+
+    aterm = 0.5 * (dFsk[4] - amx * dqsk[4]);
+    bterm = 0.5 * (dFsk[3] - amx * dqsk[3]);
+    cterm = 0.5 * (dFsk[2] - amx * dqsk[2]);
+    dterm = 0.5 * (dFsk[1] - amx * dqsk[1]);
+    IS0 = 13.0 * (aterm - bterm) * (aterm - bterm) + 3.0 * (aterm - 3.0 * bterm) * (aterm - 3.0 * bterm);
+    IS1 = 13.0 * (bterm - cterm) * (bterm - cterm) + 3.0 * (bterm + cterm) * (bterm + cterm);
+    IS2 = 13.0 * (cterm - dterm) * (cterm - dterm) + 3.0 * (3.0 * cterm - dterm) * (3.0 * cterm - dterm);
+    alpha0 = 1.0 / ((eps + IS0) * (eps + IS0));
+    alpha1 = 6.0 / ((eps + IS1) * (eps + IS1));
+    alpha2 = 3.0 / ((eps + IS2) * (eps + IS2));
+    omega0 = alpha0 / (alpha0 + alpha1 + alpha2);
+    omega2 = alpha2 / (alpha0 + alpha1 + alpha2);
+    if (wm) { wm[0] = omega0; wm[1] = alpha1 / (alpha0 + alpha1 + alpha2); wm[2] = omega2; }
+    const double third = omega0 * (aterm - 2.0 * bterm + cterm) / 3.0
+                       + (omega2 - 0.5) * (bterm - 2.0 * cterm + dterm) / 6.0;
+    return first - second + third;
+}
+
+/* weno_x_all, src/ssprk.cpp:825-997: flux (serial wave order), update and CT terms */
+static void line_weno(line_ws* w, double dtdl) {
+    const int nn = w->nn;
+    double Fsk[6], qsk[6];
+    memset(w->dF, 0, sizeof(double) * 7 * (size_t)(nn + 1));
+    for (int i = 0; i <= nn; ++i) {
+        for (int m = 0; m < 7; ++m) {
+            double amx = 0.0;
+            for (int s = -2; s <= 3; ++s) amx = fmax(amx, fabs(WA(w, m, i + s + 2)));
+            for (int ks = 0; ks < 6; ++ks) {
+                const int c = i + ks;
+                Fsk[ks] = WL(w, m, 0, i + 2) * WF(w, 0, c) + WL(w, m, 1, i + 2) * WF(w, 1, c)
+                        + WL(w, m, 2, i + 2) * WF(w, 2, c) + WL(w, m, 3, i + 2) * WF(w, 3, c)
+                        + WL(w, m, 4, i + 2) * WF(w, 4, c) + WL(w, m, 5, i + 2) * WF(w, 5, c)
+                        + WL(w, m, 6, i + 2) * WF(w, 6, c);
+                qsk[ks] = WL(w, m, 0, i + 2) * WQ(w, 0, c) + WL(w, m, 1, i + 2) * WQ(w, 1, c)
+                        + WL(w, m, 2, i + 2) * WQ(w, 2, c) + WL(w, m, 3, i + 2) * WQ(w, 3, c)
+                        + WL(w, m, 4, i + 2) * WQ(w, 5, c) + WL(w, m, 5, i + 2) * WQ(w, 6, c)
+                        + WL(w, m, 6, i + 2) * WQ(w, 7, c);
+            }
+            const double Fs = weno_wave(Fsk, qsk, amx, NULL, NULL);
+            for (int r = 0; r < 7; ++r) WD(w, r, i) += WR(w, r, m, i + 2) * Fs;
+        }
+    }
+    /* weno_x_all_update, 949-965 (reads dF only, overwrites qx) */
+    for (int i = 1; i <= nn; ++i) {
+        WQ(w, 0, i + 2) = -dtdl * (WD(w, 0, i) - WD(w, 0, i - 1));
+        WQ(w, 1, i + 2) = -dtdl * (WD(w, 1, i) - WD(w, 1, i - 1));
+        WQ(w, 2, i + 2) = -dtdl * (WD(w, 2, i) - WD(w, 2, i - 1));
+        WQ(w, 3, i + 2) = -dtdl * (WD(w, 3, i) - WD(w, 3, i - 1));
+        WQ(w, 4, i + 2) = 0.0;
+        WQ(w, 5, i + 2) = -dtdl * (WD(w, 4, i) - WD(w, 4, i - 1));
+        WQ(w, 6, i + 2) = -dtdl * (WD(w, 5, i) - WD(w, 5, i - 1));
+        WQ(w, 7, i + 2) = -dtdl * (WD(w, 6, i) - WD(w, 6, i - 1));
+    }
+    /* weno_x_all_fluxct_terms, 967-994 */
+    for (int i = 0; i <= nn; ++i) {
+        const int im1 = i + 1, i0 = i + 2, ip1 = i + 3, ip2 = i + 4;
+        const double ftmp1 = (-0.0 * (WU(w, 4, im1) * WU(w, 2, im1)) + 9.0 * (WU(w, 4, i0) * WU(w, 2, i0))
+                            + 9.0 * (WU(w, 4, ip1) * WU(w, 2, ip1)) - 0.0 * (WU(w, 4, ip2) * WU(w, 2, ip2))) / 18.0;
+        const double ftmp2 = (-0.0 * (WU(w, 4, im1) * WU(w, 3, im1)) + 9.0 * (WU(w, 4, i0) * WU(w, 3, i0))
+                            + 9.0 * (WU(w, 4, ip1) * WU(w, 3, ip1)) - 0.0 * (WU(w, 4, ip2) * WU(w, 3, ip2))) / 18.0;
+        w->bsy[i] = WD(w, 4, i) + ftmp1;
+        w->bsz[i] = WD(w, 5, i) + ftmp2;
+    }
+}
+
+static void line_pipeline(line_ws* w, double gam, double rhomin, double pgmin, double dtdl) {
+    line_primit(w, gam, rhomin, pgmin);
+    line_center(w, gam);
+    line_boundary(w, gam);
+    line_weno(w, dtdl);
+}
+
+/* =================================================== directional sweeps ==== */
+/* Variable permutations of load_{x,y,z}_all_lines (src/ssprk.cpp:381-403,
+ * 1035-1065, 1126-1156): workspace slot m holds q variable perm[d][m]. */
+static const int PERM[3][8] = {
+    {0, 1, 2, 3, 4, 5, 6, 7},
+    {0, 2, 3, 1, 5, 6, 4, 7},
+    {0, 3, 1, 2, 6, 4, 5, 7},
+};
+
+/* x_sweep / y_sweep / z_sweep, src/ssprk.cpp:1000-1025, 1218-1272, with the
+ * store kernels 405-456, 1067-1124, 1158-1215. */
+static void sweep(howmhd_ref_ctx* c, int dir, int iw, rkc k, double dtdl) {
+    const int nd[3] = {c->p.nx, c->p.ny, c->p.nz};
+    const int n6[3] = {c->n6x, c->n6y, c->n6z};
+    const int nn = nd[dir];
+    const int ta = (dir == 0) ? 1 : 0;          /* first transverse axis  */
+    const int tb = (dir == 2) ? 1 : 2;          /* second transverse axis */
+    const int na = n6[ta], nb = n6[tb];
+    double* ct_y = c->ct[dir == 0 ? FSY : (dir == 1 ? GSZ : HSX)];
+    double* ct_z = c->ct[dir == 0 ? FSZ : (dir == 1 ? GSX : HSY)];
+    const double gam = c->p.gam, rhomin = c->p.rhomin, pgmin = c->p.pgmin;
+
+#pragma omp parallel
+    {
+        line_ws w;
+        int ok = ws_alloc(&w, nn);
+#pragma omp for collapse(2) schedule(static)
+        for (int b = 0; b < nb; ++b)
+            for (int a = 0; a < na; ++a) {
+                if (ok != 0) continue;
+                int idx[3];
+                idx[ta] = a;
+                idx[tb] = b;
+                for (int m = 0; m < 8; ++m)
+                    for (int s = 0; s < nn + 6; ++s) {
+                        idx[dir] = s;
+                        WQ(&w, m, s) = Q(c, iw - 1, PERM[dir][m], idx[0], idx[1], idx[2]);
+                    }
+                /* L and R are read only at interfaces 2..nn+2, all rewritten per line */
+                line_pipeline(&w, gam, rhomin, pgmin, dtdl);
+                for (int i = 0; i <= nn; ++i) {
+                    idx[dir] = i + 2;
+                    A3(c, ct_y, idx[0], idx[1], idx[2]) = w.bsy[i];
+                    A3(c, ct_z, idx[0], idx[1], idx[2]) = w.bsz[i];
+                }
+                for (int m = 0; m < 8; ++m)
+                    for (int s = 3; s < nn + 3; ++s) {
+                        idx[dir] = s;
+                        double* dst = &Q(c, iw, PERM[dir][m], idx[0], idx[1], idx[2]);
+                        if (dir == 0)
+                            *dst = k.k1 * *dst + k.k3 * Q(c, iw - 1, m, idx[0], idx[1], idx[2]) + WQ(&w, m, s);
+                        else
+                            *dst += WQ(&w, m, s);
+                    }
+            }
+        ws_free(&w);
+    }
+}
+
+/* ================================================================ fluxct === */
+/* bc_index_ct, src/fluxct.cpp:14-25 */
+static int bci(int idx, int lo, int hi, int periodic) {
+    if (periodic) {
+        const int n = hi - lo + 1;
+        int r = (idx - lo) % n;
+        if (r < 0) r += n;
+        return lo + r;
+    }
+    if (idx < lo) return lo;
+    if (idx > hi) return hi;
+    return idx;
+}
+
+/* fluxct, src/fluxct.cpp:786-899 (serial boundary paths) */
+static void fluxct(howmhd_ref_ctx* c, int iw, double k1, double k3) {
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const int use_x = nx > 1, use_y = ny > 1, use_z = nz > 1;
+    const int use_Ox = use_x && use_y, use_Oy = use_y && use_z, use_Oz = use_x && use_z;
+    if (!use_Ox && !use_Oy && !use_Oz) return;
+    const int xp = c->p.bc[0] == HOWMHD_BC_PERIODIC;
+    const int yp = c->p.bc[1] == HOWMHD_BC_PERIODIC;
+    const int zp = c->p.bc[2] == HOWMHD_BC_PERIODIC;
+    const int hx = nx + 5, hy = ny + 5, hz = nz + 5;
+    double* const* a = c->ct;
+
+    for (int t = FSY; t <= HSY; ++t) bound_a3(c, c->ct[t]);       /* bound_ct_fluxes, 765-772 */
+
+    /* fluxct_build_O2, 809-821 */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i <= nx + 3; ++i)
+        for (int j = 0; j <= ny + 3; ++j)
+            for (int k = 0; k <= nz + 3; ++k) {
+                const int im1 = bci(i - 1, 0, hx, xp), ip1 = bci(i + 1, 0, hx, xp), ip2 = bci(i + 2, 0, hx, xp);
+                const int jm1 = bci(j - 1, 0, hy, yp), jp1 = bci(j + 1, 0, hy, yp), jp2 = bci(j + 2, 0, hy, yp);
+                const int km1 = bci(k - 1, 0, hz, zp), kp1 = bci(k + 1, 0, hz, zp), kp2 = bci(k + 2, 0, hz, zp);
+                if (use_Ox)
+                    A3(c, a[OX2], i, j, k) =
+                        (-A3(c, a[GSX], im1, j, k) + 9.0 * A3(c, a[GSX], i, j, k) + 9.0 * A3(c, a[GSX], ip1, j, k) - A3(c, a[GSX], ip2, j, k)) / 16.0
+                      - (-A3(c, a[FSY], i, jm1, k) + 9.0 * A3(c, a[FSY], i, j, k) + 9.0 * A3(c, a[FSY], i, jp1, k) - A3(c, a[FSY], i, jp2, k)) / 16.0;
+                if (use_Oy)
+                    A3(c, a[OY2], i, j, k) =
+                        (-A3(c, a[HSY], i, jm1, k) + 9.0 * A3(c, a[HSY], i, j, k) + 9.0 * A3(c, a[HSY], i, jp1, k) - A3(c, a[HSY], i, jp2, k)) / 16.0
+                      - (-A3(c, a[GSZ], i, j, km1) + 9.0 * A3(c, a[GSZ], i, j, k) + 9.0 * A3(c, a[GSZ], i, j, kp1) - A3(c, a[GSZ], i, j, kp2)) / 16.0;
+                if (use_Oz)
+                    A3(c, a[OZ2], i, j, k) =
+                        (-A3(c, a[FSZ], i, j, km1) + 9.0 * A3(c, a[FSZ], i, j, k) + 9.0 * A3(c, a[FSZ], i, j, kp1) - A3(c, a[FSZ], i, j, kp2)) / 16.0
+                      - (-A3(c, a[HSX], im1, j, k) + 9.0 * A3(c, a[HSX], i, j, k) + 9.0 * A3(c, a[HSX], ip1, j, k) - A3(c, a[HSX], ip2, j, k)) / 16.0;
+            }
+    for (int t = OX2; t <= OZ2; ++t) bound_a3(c, c->ct[t]);       /* bound_ct_emf2 */
+
+    /* fluxct_smooth_O, 828-840 */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= nx + 4; ++i)
+        for (int j = 1; j <= ny + 4; ++j)
+            for (int k = 1; k <= nz + 4; ++k) {
+                const int im1 = bci(i - 1, 0, hx, xp), ip1 = bci(i + 1, 0, hx, xp);
+                const int jm1 = bci(j - 1, 0, hy, yp), jp1 = bci(j + 1, 0, hy, yp);
+                const int km1 = bci(k - 1, 0, hz, zp), kp1 = bci(k + 1, 0, hz, zp);
+                if (use_Ox) {
+                    const double o = A3(c, a[OX2], i, j, k);
+                    A3(c, a[OX], i, j, k) = o + (A3(c, a[OX2], im1, j, k) - 2.0 * o + A3(c, a[OX2], ip1, j, k)) / 24.0
+                                              + (A3(c, a[OX2], i, jm1, k) - 2.0 * o + A3(c, a[OX2], i, jp1, k)) / 24.0;
+                }
+                if (use_Oy) {
+                    const double o = A3(c, a[OY2], i, j, k);
+                    A3(c, a[OY], i, j, k) = o + (A3(c, a[OY2], i, jm1, k) - 2.0 * o + A3(c, a[OY2], i, jp1, k)) / 24.0
+                                              + (A3(c, a[OY2], i, j, km1) - 2.0 * o + A3(c, a[OY2], i, j, kp1)) / 24.0;
+                }
+                if (use_Oz) {
+                    const double o = A3(c, a[OZ2], i, j, k);
+                    A3(c, a[OZ], i, j, k) = o + (A3(c, a[OZ2], im1, j, k) - 2.0 * o + A3(c, a[OZ2], ip1, j, k)) / 24.0
+                                              + (A3(c, a[OZ2], i, j, km1) - 2.0 * o + A3(c, a[OZ2], i, j, kp1)) / 24.0;
+                }
+            }
+    for (int t = OX; t <= OZ; ++t) bound_a3(c, c->ct[t]);         /* bound_ct_emf */
+
+    /* fluxct_update_bfield, 848-878 */
+    const double aa = 9.0 / 8.0, bb = -1.0 / 24.0, cc = 0.0;
+    const double dtdx = c->dtdx, dtdy = c->dtdy, dtdz = c->dtdz;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= nx + 3; ++i)
+        for (int j = 1; j <= ny + 3; ++j)
+            for (int k = 1; k <= nz + 3; ++k) {
+                const int im1 = bci(i - 1, 0, hx, xp), im2 = bci(i - 2, 0, hx, xp), im3 = bci(i - 3, 0, hx, xp);
+                const int ip1 = bci(i + 1, 0, hx, xp), ip2 = bci(i + 2, 0, hx, xp);
+                const int jm1 = bci(j - 1, 0, hy, yp), jm2 = bci(j - 2, 0, hy, yp), jm3 = bci(j - 3, 0, hy, yp);
+                const int jp1 = bci(j + 1, 0, hy, yp), jp2 = bci(j + 2, 0, hy, yp);
+                const int km1 = bci(k - 1, 0, hz, zp), km2 = bci(k - 2, 0, hz, zp), km3 = bci(k - 3, 0, hz, zp);
+                const int kp1 = bci(k + 1, 0, hz, zp), kp2 = bci(k + 2, 0, hz, zp);
+                const double* Ox = a[OX];
+                const double* Oy = a[OY];
+                const double* Oz = a[OZ];
+                if (use_Ox || use_Oz) {
+                    double dOx = 0.0, dOz = 0.0;
+                    if (use_Ox) dOx = aa * (A3(c, Ox, i, j, k) - A3(c, Ox, i, jm1, k)) + bb * (A3(c, Ox, i, jp1, k) - A3(c, Ox, i, jm2, k)) + cc * (A3(c, Ox, i, jp2, k) - A3(c, Ox, i, jm3, k));
+                    if (use_Oz) dOz = aa * (A3(c, Oz, i, j, k) - A3(c, Oz, i, j, km1)) + bb * (A3(c, Oz, i, j, kp1) - A3(c, Oz, i, j, km2)) + cc * (A3(c, Oz, i, j, kp2) - A3(c, Oz, i, j, km3));
+                    BX(c, iw, i, j, k) = k1 * BX(c, iw, i, j, k) + k3 * BX(c, iw - 1, i, j, k) - dtdy * dOx + dtdz * dOz;
+                }
+                if (use_Oy || use_Ox) {
+                    double dOy = 0.0, dOx = 0.0;
+                    if (use_Oy) dOy = aa * (A3(c, Oy, i, j, k) - A3(c, Oy, i, j, km1)) + bb * (A3(c, Oy, i, j, kp1) - A3(c, Oy, i, j, km2)) + cc * (A3(c, Oy, i, j, kp2) - A3(c, Oy, i, j, km3));
+                    if (use_Ox) dOx = aa * (A3(c, Ox, i, j, k) - A3(c, Ox, im1, j, k)) + bb * (A3(c, Ox, ip1, j, k) - A3(c, Ox, im2, j, k)) + cc * (A3(c, Ox, ip2, j, k) - A3(c, Ox, im3, j, k));
+                    BY(c, iw, i, j, k) = k1 * BY(c, iw, i, j, k) + k3 * BY(c, iw - 1, i, j, k) - dtdz * dOy + dtdx * dOx;
+                }
+                if (use_Oz || use_Oy) {
+                    double dOz = 0.0, dOy = 0.0;
+                    if (use_Oz) dOz = aa * (A3(c, Oz, i, j, k) - A3(c, Oz, im1, j, k)) + bb * (A3(c, Oz, ip1, j, k) - A3(c, Oz, im2, j, k)) + cc * (A3(c, Oz, ip2, j, k) - A3(c, Oz, im3, j, k));
+                    if (use_Oy) dOy = aa * (A3(c, Oy, i, j, k) - A3(c, Oy, i, jm1, k)) + bb * (A3(c, Oy, i, jp1, k) - A3(c, Oy, i, jm2, k)) + cc * (A3(c, Oy, i, jp2, k) - A3(c, Oy, i, jm3, k));
+                    BZ(c, iw, i, j, k) = k1 * BZ(c, iw, i, j, k) + k3 * BZ(c, iw - 1, i, j, k) - dtdx * dOz + dtdy * dOy;
+                }
+            }
+    bound_faces(c, iw);                                            /* 880 */
+
+    /* fluxct_interp_b_to_q, 882-897 */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 3; i <= nx + 2; ++i)
+        for (int j = 3; j <= ny + 2; ++j)
+            for (int k = 3; k <= nz + 2; ++k) {
+                const int im1 = bci(i - 1, 0, hx, xp), im2 = bci(i - 2, 0, hx, xp), im3 = bci(i - 3, 0, hx, xp);
+                const int ip1 = bci(i + 1, 0, hx, xp), ip2 = bci(i + 2, 0, hx, xp);
+                const int jm1 = bci(j - 1, 0, hy, yp), jm2 = bci(j - 2, 0, hy, yp), jm3 = bci(j - 3, 0, hy, yp);
+                const int jp1 = bci(j + 1, 0, hy, yp), jp2 = bci(j + 2, 0, hy, yp);
+                const int km1 = bci(k - 1, 0, hz, zp), km2 = bci(k - 2, 0, hz, zp), km3 = bci(k - 3, 0, hz, zp);
+                const int kp1 = bci(k + 1, 0, hz, zp), kp2 = bci(k + 2, 0, hz, zp);
+                if (use_x)
+                    Q(c, iw, 4, i, j, k) = (3.0 * BX(c, iw, im3, j, k) - 25.0 * BX(c, iw, im2, j, k) + 150.0 * BX(c, iw, im1, j, k)
+                                          + 150.0 * BX(c, iw, i, j, k) - 25.0 * BX(c, iw, ip1, j, k) + 3.0 * BX(c, iw, ip2, j, k)) / 256.0;
+                else
+                    Q(c, iw, 4, i, j, k) = BX(c, iw, i, j, k);
+                if (use_y)
+                    Q(c, iw, 5, i, j, k) = (3.0 * BY(c, iw, i, jm3, k) - 25.0 * BY(c, iw, i, jm2, k) + 150.0 * BY(c, iw, i, jm1, k)
+                                          + 150.0 * BY(c, iw, i, j, k) - 25.0 * BY(c, iw, i, jp1, k) + 3.0 * BY(c, iw, i, jp2, k)) / 256.0;
+                else
+                    Q(c, iw, 5, i, j, k) = BY(c, iw, i, j, k);
+                if (use_z)
+                    Q(c, iw, 6, i, j, k) = (3.0 * BZ(c, iw, i, j, km3) - 25.0 * BZ(c, iw, i, j, km2) + 150.0 * BZ(c, iw, i, j, km1)
+                                          + 150.0 * BZ(c, iw, i, j, k) - 25.0 * BZ(c, iw, i, j, kp1) + 3.0 * BZ(c, iw, i, j, kp2)) / 256.0;
+                else
+                    Q(c, iw, 6, i, j, k) = BZ(c, iw, i, j, k);
+            }
+}
+
+/* ================================================================== prot === */
+/* src/prot.cpp:6-54 */
+static void prot(howmhd_ref_ctx* c, int s) {
+    const double gam = c->p.gam, rhomin = c->p.rhomin, pgmin = c->p.pgmin;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 3; i < c->p.nx + 3; ++i)
+        for (int j = 3; j < c->p.ny + 3; ++j)
+            for (int k = 3; k < c->p.nz + 3; ++k) {
+                double rho = Q(c, s, 0, i, j, k);
+                const double Mx = Q(c, s, 1, i, j, k), My = Q(c, s, 2, i, j, k), Mz = Q(c, s, 3, i, j, k);
+                const double Bx = Q(c, s, 4, i, j, k), By = Q(c, s, 5, i, j, k), Bz = Q(c, s, 6, i, j, k);
+                double E = Q(c, s, 7, i, j, k);
+                rho = fmax(rho, rhomin);
+                const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
+                const double vv2 = vx * vx + vy * vy + vz * vz;
+                const double BB2 = Bx * Bx + By * By + Bz * Bz;
+                double pg = (gam - 1.0) * (E - 0.5 * (rho * vv2 + BB2));
+                pg = fmax(pg, pgmin);
+                E = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
+                Q(c, s, 0, i, j, k) = rho;
+                Q(c, s, 7, i, j, k) = E;
+            }
+}
+
+/* ================================================================= ssprk === */
+static void copy_stage(howmhd_ref_ctx* c, int dst, int src) {                 /* 99-116 */
+    memcpy(&c->q[(size_t)dst * NVAR * c->ncell], &c->q[(size_t)src * NVAR * c->ncell],
+           sizeof(double) * NVAR * c->ncell);
+}
+
+static void prepare_stage5(howmhd_ref_ctx* c, rkc k) {                          /* 121-140 */
+    const size_t n = NVAR * c->ncell;
+    double* q0 = c->q;
+    double* q2 = c->q + 2 * n;
+    double* q3 = c->q + 3 * n;
+    double* q5 = c->q + 5 * n;
+#pragma omp parallel for schedule(static)
+    for (size_t t = 0; t < n; ++t) q5[t] = q0[t] + (k.k4 / k.k1) * q2[t] + (k.k5 / k.k1) * q3[t];
+}
+
+static void copy_faces(howmhd_ref_ctx* c, int dst, int src) {                 /* fluxct.cpp:111-122 */
+    const size_t n = c->ncell;
+    memcpy(c->bx + dst * n, c->bx + src * n, sizeof(double) * n);
+    memcpy(c->by + dst * n, c->by + src * n, sizeof(double) * n);
+    memcpy(c->bz + dst * n, c->bz + src * n, sizeof(double) * n);
+}
+
+static void prepare_faces5(howmhd_ref_ctx* c, rkc k) {                          /* fluxct.cpp:124-135 */
+    const size_t n = c->ncell;
+    double* arr[3] = {c->bx, c->by, c->bz};
+    for (int d = 0; d < 3; ++d) {
+        double* b = arr[d];
+#pragma omp parallel for schedule(static)
+        for (size_t t = 0; t < n; ++t) b[5 * n + t] = b[t] + (k.k4 / k.k1) * b[2 * n + t] + (k.k5 / k.k1) * b[3 * n + t];
+    }
+}
+
+int howmhd_ref_ssprk(howmhd_ref_ctx* c, double dt) {
+    c->dt = dt;
+    for (int iw = 1; iw <= 5; ++iw) {
+        for (int t = 0; t < 12; ++t) memset(c->ct[t], 0, sizeof(double) * c->ncell);  /* zero_ct_fluxes */
+        const rkc k = rk_coeff(iw);
+        if (iw <= 4) {
+            copy_stage(c, iw, 0);
+            copy_faces(c, iw, 0);
+        } else {
+            prepare_stage5(c, k);
+            prepare_faces5(c, k);
+        }
+        bound_q(c, iw - 1);
+        if (iw == 5) bound_q(c, iw);
+        c->dtdx = k.k2 * c->dt / c->p.dx;
+        c->dtdy = k.k2 * c->dt / c->p.dy;
+        c->dtdz = k.k2 * c->dt / c->p.dz;
+        if (c->p.nx > 1) sweep(c, 0, iw, k, c->dtdx);
+        if (c->p.ny > 1) sweep(c, 1, iw, k, c->dtdy);
+        if (c->p.nz > 1) sweep(c, 2, iw, k, c->dtdz);
+        bound_faces(c, iw - 1);
+        fluxct(c, iw, k.k1, k.k3);
+        prot(c, iw);
+        bound_q(c, iw);
+        bound_faces(c, iw);
+    }
+    copy_stage(c, 0, 5);
+    copy_faces(c, 0, 5);
+    return 0;
+}
+
+/* ================================================================= tstep === */
+/* src/tstep.cpp:11-85, 123-135 (one rank: the Allreduce is the identity) */
+int howmhd_ref_tstep(howmhd_ref_ctx* c, double* dt_out) {
+    const double gam = c->p.gam, rhomin = c->p.rhomin, pgmin = c->p.pgmin;
+    double vxmax = 0.0, vymax = 0.0, vzmax = 0.0;
+#pragma omp parallel
+    {
+        double mx = 0.0, my = 0.0, mz = 0.0;
+#pragma omp for collapse(2) schedule(static)
+        for (int i = 3; i < c->p.nx + 3; ++i)
+            for (int j = 3; j < c->p.ny + 3; ++j)
+                for (int k = 3; k < c->p.nz + 3; ++k) {
+                    const double DD = Q(c, 0, 0, i, j, k), Mx = Q(c, 0, 1, i, j, k), My = Q(c, 0, 2, i, j, k), Mz = Q(c, 0, 3, i, j, k);
+                    const double Bx = Q(c, 0, 4, i, j, k), By = Q(c, 0, 5, i, j, k), Bz = Q(c, 0, 6, i, j, k), EE = Q(c, 0, 7, i, j, k);
+                    const double rho = fmax(DD, rhomin);
+                    const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
+                    const double vv2 = vx * vx + vy * vy + vz * vz;
+                    const double BB2 = Bx * Bx + By * By + Bz * Bz;
+                    const double pg = fmax(pgmin, (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2)));
+                    const double bbn2 = BB2 / rho;
+                    const double cs2 = fmax(0.0, gam * pg / rho);
+                    const double bnx2 = Bx * Bx / rho, bny2 = By * By / rho, bnz2 = Bz * Bz / rho;
+                    const double rootx = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+                    const double rooty = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bny2 * cs2));
+                    const double rootz = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnz2 * cs2));
+                    const double lfx = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootx)));
+                    const double lfy = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rooty)));
+                    const double lfz = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootz)));
+                    mx = fmax(mx, fabs(vx) + lfx);
+                    my = fmax(my, fabs(vy) + lfy);
+                    mz = fmax(mz, fabs(vz) + lfz);
+                }
+#pragma omp critical
+        {
+            vxmax = fmax(vxmax, mx);
+            vymax = fmax(vymax, my);
+            vzmax = fmax(vzmax, mz);
+        }
+    }
+    const double tiny = 1.0e-300;
+    double invdt = 0.0;
+    if (c->p.nx > 1) invdt += fmax(vxmax, tiny) / c->p.dx;
+    if (c->p.ny > 1) invdt += fmax(vymax, tiny) / c->p.dy;
+    if (c->p.nz > 1) invdt += fmax(vzmax, tiny) / c->p.dz;
+    if (invdt <= 0.0) invdt = fmax(vxmax, tiny) / c->p.dx;
+    c->dt = c->p.cour / invdt;
+    *dt_out = c->dt;
+    return 0;
+}
+
+/* ============================================================ init faces === */
+/* bound(q,0) + initialize_bfield_from_q + bound_bfield(0),
+ * src/main.cpp:127-130, src/fluxct.cpp:65-107 */
+int howmhd_ref_init_faces(howmhd_ref_ctx* c) {
+    bound_q(c, 0);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= c->p.nx + 3; ++i)
+        for (int j = 1; j <= c->p.ny + 3; ++j)
+            for (int k = 1; k <= c->p.nz + 3; ++k) {
+                BX(c, 0, i, j, k) = (-Q(c, 0, 4, i - 1, j, k) + 9.0 * Q(c, 0, 4, i, j, k) + 9.0 * Q(c, 0, 4, i + 1, j, k) - Q(c, 0, 4, i + 2, j, k)) / 16.0;
+                BY(c, 0, i, j, k) = (-Q(c, 0, 5, i, j - 1, k) + 9.0 * Q(c, 0, 5, i, j, k) + 9.0 * Q(c, 0, 5, i, j + 1, k) - Q(c, 0, 5, i, j + 2, k)) / 16.0;
+                BZ(c, 0, i, j, k) = (-Q(c, 0, 6, i, j, k - 1) + 9.0 * Q(c, 0, 6, i, j, k) + 9.0 * Q(c, 0, 6, i, j, k + 1) - Q(c, 0, 6, i, j, k + 2)) / 16.0;
+            }
+    bound_faces(c, 0);
+    return 0;
+}
+
+/* ============================================================ lifecycle ==== */
+int howmhd_ref_create(const howmhd_params* p, howmhd_ref_ctx** out) {
+    if (!p || !out) return HOWMHD_EINVAL;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return HOWMHD_EINVAL;
+    howmhd_ref_ctx* c = (howmhd_ref_ctx*)calloc(1, sizeof(*c));
+    if (!c) return HOWMHD_ENOMEM;
+    c->p = *p;
+    c->n6x = p->nx + 6; c->n6y = p->ny + 6; c->n6z = p->nz + 6;
+    c->ncell = (size_t)c->n6x * c->n6y * c->n6z;
+    c->q = (double*)calloc((size_t)NSTAGE * NVAR * c->ncell, sizeof(double));
+    c->bx = (double*)calloc((size_t)NSTAGE * c->ncell, sizeof(double));
+    c->by = (double*)calloc((size_t)NSTAGE * c->ncell, sizeof(double));
+    c->bz = (double*)calloc((size_t)NSTAGE * c->ncell, sizeof(double));
+    int ok = c->q && c->bx && c->by && c->bz;
+    for (int t = 0; t < 12; ++t) {
+        c->ct[t] = (double*)calloc(c->ncell, sizeof(double));
+        ok = ok && c->ct[t];
+    }
+    if (!ok) { howmhd_ref_destroy(c); return HOWMHD_ENOMEM; }
+    *out = c;
+    return 0;
+}
+
+int howmhd_ref_destroy(howmhd_ref_ctx* c) {
+    if (!c) return 0;
+    free(c->q); free(c->bx); free(c->by); free(c->bz);
+    for (int t = 0; t < 12; ++t) free(c->ct[t]);
+    free(c);
+    return 0;
+}
+
+int howmhd_ref_upload_q0(howmhd_ref_ctx* c, const double* q0) {
+    memcpy(c->q, q0, sizeof(double) * NVAR * c->ncell);
+    return 0;
+}
+
+int howmhd_ref_upload_faces(howmhd_ref_ctx* c, const double* b) {
+    memcpy(c->bx, b, sizeof(double) * c->ncell);
+    memcpy(c->by, b + c->ncell, sizeof(double) * c->ncell);
+    memcpy(c->bz, b + 2 * c->ncell, sizeof(double) * c->ncell);
+    bound_faces(c, 0);
+    return 0;
+}
+
+int howmhd_ref_download_q0(howmhd_ref_ctx* c, double* q0) {
+    memcpy(q0, c->q, sizeof(double) * NVAR * c->ncell);
+    return 0;
+}
+
+int howmhd_ref_download_faces(howmhd_ref_ctx* c, double* b) {
+    memcpy(b, c->bx, sizeof(double) * c->ncell);
+    memcpy(b + c->ncell, c->by, sizeof(double) * c->ncell);
+    memcpy(b + 2 * c->ncell, c->bz, sizeof(double) * c->ncell);
+    return 0;
+}
+
+int howmhd_ref_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+/* ====================================================== test entry points == */
+int howmhd_ref_line_sweep(double gam, double rhomin, double pgmin, int nn, double dtdl,
+                          double* qline, double* bsy, double* bsz, double* dF_out) {
+    line_ws w;
+    if (nn < 1 || ws_alloc(&w, nn) != 0) return HOWMHD_EINVAL;
+    memcpy(w.qx, qline, sizeof(double) * 8 * (size_t)w.N);
+    line_pipeline(&w, gam, rhomin, pgmin, dtdl);
+    memcpy(qline, w.qx, sizeof(double) * 8 * (size_t)w.N);
+    if (bsy) memcpy(bsy, w.bsy, sizeof(double) * (size_t)(nn + 1));
+    if (bsz) memcpy(bsz, w.bsz, sizeof(double) * (size_t)(nn + 1));
+    if (dF_out) memcpy(dF_out, w.dF, sizeof(double) * 7 * (size_t)(nn + 1));
+    ws_free(&w);
+    return 0;
+}
+
+void howmhd_ref_eigensystem(double gam, const double* ul, const double* ur, double* L, double* R) {
+    eigensystem(gam, ul, ur, L, R);
+}
+
+double howmhd_ref_weno_wave(const double* Fsk, const double* qsk, double amx) {
+    return weno_wave(Fsk, qsk, amx, NULL, NULL);
+}
+
+double howmhd_ref_weno_wave_weights(const double* Fsk, const double* qsk, double amx,
+                                    double* wplus, double* wminus) {
+    return weno_wave(Fsk, qsk, amx, wplus, wminus);
+}

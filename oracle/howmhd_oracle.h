@@ -1,0 +1,71 @@
+/*
+ * howmhd_oracle.h — CPU restatement of the HOW-MHD time step (TEST
+ * INFRASTRUCTURE ONLY).
+ *
+ * This library is the parity checker and the timed CPU baseline.  Only the
+ * test suite, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it;
+ * the product (the HIP library and the host driver) never does.
+ *
+ * It restates, in plain C with the reference's operation order, the serial
+ * (one-rank) paths of:
+ *   src/ssprk.cpp    54-140, 381-1329 (live code; dF summed in wave order 0..6)
+ *   src/fluxct.cpp   14-285, 481-552, 753-899
+ *   src/bound.cpp    19-107, 295-317
+ *   src/tstep.cpp    11-85, 123-135
+ *   src/prot.cpp     10-51
+ * State is kept exactly as the reference keeps it: q(stage,var,i,j,k) with
+ * 3 ghost cells per side (k fastest), 6 stages, face arrays bxb/byb/bzb(stage,
+ * i,j,k) and the 12 CT work arrays.
+ */
+#ifndef HOWMHD_ORACLE_H
+#define HOWMHD_ORACLE_H
+
+#include "../include/howmhd_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct howmhd_ref_ctx howmhd_ref_ctx;
+
+int howmhd_ref_create(const howmhd_params* p, howmhd_ref_ctx** out);
+int howmhd_ref_destroy(howmhd_ref_ctx* c);
+int howmhd_ref_upload_q0(howmhd_ref_ctx* c, const double* q0);
+int howmhd_ref_upload_faces(howmhd_ref_ctx* c, const double* b);
+int howmhd_ref_init_faces(howmhd_ref_ctx* c);
+int howmhd_ref_tstep(howmhd_ref_ctx* c, double* dt);
+int howmhd_ref_ssprk(howmhd_ref_ctx* c, double dt);
+int howmhd_ref_download_q0(howmhd_ref_ctx* c, double* q0);
+int howmhd_ref_download_faces(howmhd_ref_ctx* c, double* b);
+int howmhd_ref_set_threads(int n);
+
+/* --- single-line and single-interface pieces, for the oracle's own checks --- */
+
+/* One directional sweep of one line (src/ssprk.cpp:458-997): qline holds
+ * 8 x (nn+6) conserved values in sweep order (rho, Mn, Mt1, Mt2, Bn, Bt1, Bt2,
+ * E).  On return qline[:, 3..nn+2] holds the increments -dtdl*(dF(i)-dF(i-1))
+ * (normal-B slot 0), and bsy/bsz (nn+1 each) the CT flux terms. */
+int howmhd_ref_line_sweep(double gam, double rhomin, double pgmin, int nn,
+                          double dtdl, double* qline, double* bsy, double* bsz,
+                          double* dF_out /* 7 x (nn+1), may be NULL */);
+
+/* Interface eigensystem between two primitive states
+ * (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H) (src/ssprk.cpp:602-819).
+ * L and R are 7x7 row-major: L[row=wave][col], R[row=variable][col=wave]. */
+void howmhd_ref_eigensystem(double gam, const double* ul, const double* ur,
+                            double* L, double* R);
+
+/* Characteristic flux of one wave from its 6 projected values
+ * (src/ssprk.cpp:883-937). */
+double howmhd_ref_weno_wave(const double* Fsk, const double* qsk, double amx);
+
+/* Same, also returning the nonlinear weights (w0,w1,w2) of the plus and the
+ * minus side (src/ssprk.cpp:903-908, 926-931). */
+double howmhd_ref_weno_wave_weights(const double* Fsk, const double* qsk,
+                                    double amx, double* wplus, double* wminus);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
