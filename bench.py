@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Headline benchmark: zone-cycles/s of the HIP HOW-MHD time step on one MI355X.
+
+Workload: 3-D MHD blast (problem mhd_blast3d) on a 256^3 periodic unit box,
+rho=1, p=0.1 ambient and p=10 for r<0.1, B=(1/sqrt2, 1/sqrt2, 0), v=0,
+gam=5/3, cour=0.4, FP64 plain method.  One "step" is one hydro cycle: the CFL
+reduction (tstep) plus one SSPRK(5,4) step with its 15 directional sweeps,
+5 CT updates and 5 stage combinations.  q(0) and the faces are resident in
+HBM before the timed region; the per-step dt read-back is part of the loop,
+as in the reference driver.
+
+value = 256^3 * steps / wall time of the timed steps.  After the timed loop a
+short run with per-kernel HIP-event timing on the library's stream gives the
+roofline entry of the dominant kernel.  The CPU baseline is the C restatement
+of the reference time step (oracle/) on the host cores, on a bounded sample
+(the same problem at 128^3, one step).
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "how-mhd-kokkos_amd"))
+
+METRIC = "zone-cycles/s per SSPRK(5,4) step at 256³; speedup vs plain FP64 at fixed L1"
+
+# Algorithmic work per unit, 3-D FP64 (add/sub/mul/div/sqrt = 1 FLOP):
+#   directional sweep: 2,850 FLOP per cell (cell values 85, interface
+#   eigensystem 540, 7 waves x 314 WENO/projection, 31 update/CT terms)
+#   whole zone-step:   44,000 FLOP and 1,472 B of compulsory state traffic
+FLOP_SWEEP_CELL = 2850.0
+FLOP_ZONE_STEP = 44000.0
+BYTES_ZONE_STEP = 1472.0
+# bytes per cell of each memory-bound kernel (FP64 path, compulsory reads+writes)
+BYTES_CELL = {
+    "emf2": 8 * (6 + 3),          # 6 CT flux arrays in, 3 EMF candidates out
+    "emf": 8 * (3 + 3),
+    "faces": 8 * (3 + 3 + 3),     # O in, previous + initial faces in, faces out
+    "combine": 8 * (21 + 8 + 8 + 3 + 8),  # 3x7 fluxes, q_init, q_prev, faces in; q out
+    "cfl": 8 * 8,
+}
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--cpu-size", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def traffic_from_profile(kernel_key: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (profiles/traffic.txt: '<kernel> <bytes_per_launch>' lines), or None."""
+    f = ROOT / "profiles" / "traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel_key and not line.startswith("#"):
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(n: int, steps: int):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import howmhd as H
+    import oracle_ctypes as O
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    O.set_threads(threads)
+    run = H.parse(H.make_input(problem="mhd_blast3d", nx=n, ny=n, nz=n, nt=1, tend=1.0, cour=0.4,
+                               gam="1.6666666666666667"))
+    q0 = H.init_problem(run)
+    ref = O.RefSolver(H.gpu_params(run))
+    ref.upload_q0(q0)
+    ref.init_faces()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ref.ssprk(ref.tstep())
+    el = time.perf_counter() - t0
+    ref.close()
+    return {
+        "value": n ** 3 * steps / el,
+        "unit": "zone-cycles/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"mhd_blast3d {n}^3, {steps} step(s) (tstep + SSPRK(5,4)) of the C restatement of the "
+                  f"reference time step, OpenMP, {threads} threads; {el:.2f} s",
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(single-GPU time step; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+    import torch  # device plumbing only: selects the GPU and synchronises
+
+    import howmhd as H
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    n = args.size
+    run = H.parse(H.make_input(problem="mhd_blast3d", nx=n, ny=n, nz=n, nt=1, tend=1.0, cour=0.4,
+                               gam="1.6666666666666667", precision=args.precision))
+    q0 = H.init_problem(run)
+    gpu = H.GpuSolver(H.gpu_params(run))
+    gpu.upload_q0(q0)
+    gpu.init_faces()
+    del q0
+    for _ in range(args.warmup):
+        gpu.ssprk(gpu.tstep())
+    gpu.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gpu.ssprk(gpu.tstep())
+    gpu.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    zones = n ** 3
+    value = zones * args.steps / el
+
+    # per-kernel device time on the library's stream
+    gpu.set_timing(True)
+    for _ in range(args.timing_steps):
+        gpu.ssprk(gpu.tstep())
+    tim = gpu.timing()
+    gpu.set_timing(False)
+    d = gpu.diag()
+    gpu.close()
+
+    flop_peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
+    kernels = {}
+    for k, (ms, cnt) in tim.items():
+        if cnt == 0:
+            continue
+        avg_ms = ms / cnt
+        ent = {"avg_ms": round(avg_ms, 4), "launches": cnt, "ms_per_step": round(ms / args.timing_steps, 4)}
+        if k.startswith("sweep"):
+            tf = FLOP_SWEEP_CELL * zones / (avg_ms * 1e-3) / 1e12
+            ent.update(achieved_tflops=round(tf, 3), frac_fp_peak=round(tf / flop_peak, 4))
+        elif k in BYTES_CELL:
+            gbs = BYTES_CELL[k] * zones / (avg_ms * 1e-3) / 1e9
+            ent.update(achieved_gbs=round(gbs, 1), frac_hbm_peak=round(gbs / PEAK_HBM_GBS, 4))
+        kernels[k] = ent
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    dk = kernels[dom]
+    if dom.startswith("sweep"):
+        achieved = dk["achieved_tflops"]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": flop_peak, "unit": "TFLOP/s",
+                "frac": round(achieved / flop_peak, 4),
+                "traffic": traffic_from_profile(dom),
+                "kernel": f"k_sweep ({dom}); FP64 vector-ALU bound (no MFMA used: MI355X FP64 vector peak = "
+                          f"FP64 matrix peak = 78.6 TFLOP/s)",
+                "per_launch": f"{FLOP_SWEEP_CELL:.0f} FLOP/cell x {zones} cells"}
+    else:
+        achieved = dk["achieved_gbs"]
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(dom),
+                "kernel": dom}
+    whole = {
+        "fp64_tflops_effective": round(FLOP_ZONE_STEP * value / 1e12, 3),
+        "frac_fp64_roof": round(FLOP_ZONE_STEP * value / (PEAK_FP64_TFLOPS * 1e12), 4),
+        "compulsory_hbm_gbs": round(BYTES_ZONE_STEP * value / 1e9, 1),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 sweeps/CT, f64 stage sums",
+        "data": "synthetic: deterministic mhd_blast3d initial condition",
+        "config": {"workload": f"mhd_blast3d {n}^3 periodic, gam 5/3, cour 0.4, p_in 10, p_out 0.1, "
+                               f"B=(1/sqrt2,1/sqrt2,0), {args.precision} plain method",
+                   "grid": [n, n, n], "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": roof,
+        "kernels": kernels,
+        "whole_step": whole,
+        "divb_ct_max": d.max_divb_ct,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_size, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

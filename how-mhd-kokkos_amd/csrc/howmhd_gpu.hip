@@ -64,6 +64,14 @@ struct howmhd_gpu_ctx {
     void* O[3] = {};
     unsigned long long* dmax = nullptr;
     bool faces_ready = false;
+    bool timing = false;
+    struct Timed {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Timed> timed;
+    double tms[HOWMHD_K_COUNT] = {};
+    int tcount[HOWMHD_K_COUNT] = {};
     std::string err = "no error";
 };
 
@@ -79,6 +87,36 @@ struct howmhd_gpu_ctx {
 namespace {
 
 unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Brackets one launch with events when timing is on.
+struct TimedLaunch {
+    howmhd_gpu_ctx* c;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(howmhd_gpu_ctx* ctx, int k) : c(ctx), kind(k) {
+        if (c->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, c->stream);
+    }
+    ~TimedLaunch() {
+        if (a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->timed.push_back({kind, a, b});
+        }
+    }
+};
+
+void drain_timing(howmhd_gpu_ctx* c) {
+    for (auto& t : c->timed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            c->tms[t.kind] += ms;
+            c->tcount[t.kind] += 1;
+        }
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    c->timed.clear();
+}
 
 int fail(howmhd_gpu_ctx* c, int code, const std::string& msg) {
     c->err = msg;
@@ -100,6 +138,7 @@ int launch_sweeps(howmhd_gpu_ctx* c, const double* qsrc) {
         a.g = g;
         a.ph = c->ph;
         const int nif = g.per[d] ? g.n[d] : g.n[d] + 1;
+        TimedLaunch tl(c, HOWMHD_K_SWEEP_X + d);
         if (d == 0) {
             dim3 grid(blocks_for(nif, SweepTile<0>::LANES), blocks_for(g.n[1], SweepTile<0>::ROWS), g.n[2]);
             hipLaunchKernelGGL((k_sweep<R, 0>), grid, dim3(256), 0, c->stream, a);
@@ -147,14 +186,20 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
         e.Ox2 = (R*)c->O2[0]; e.Oy2 = (R*)c->O2[1]; e.Oz2 = (R*)c->O2[2];
         e.g = c->g;
         e.f = c->ctf;
-        hipLaunchKernelGGL((k_emf2<R>), dim3(nb), dim3(256), 0, c->stream, e);
+        {
+            TimedLaunch tl(c, HOWMHD_K_EMF2);
+            hipLaunchKernelGGL((k_emf2<R>), dim3(nb), dim3(256), 0, c->stream, e);
+        }
         HCHECK(c, hipGetLastError());
         SmoothArgs<R> m;
         m.Ox2 = (const R*)c->O2[0]; m.Oy2 = (const R*)c->O2[1]; m.Oz2 = (const R*)c->O2[2];
         m.Ox = (R*)c->O[0]; m.Oy = (R*)c->O[1]; m.Oz = (R*)c->O[2];
         m.g = c->g;
         m.f = c->ctf;
-        hipLaunchKernelGGL((k_emf<R>), dim3(nb), dim3(256), 0, c->stream, m);
+        {
+            TimedLaunch tl(c, HOWMHD_K_EMF);
+            hipLaunchKernelGGL((k_emf<R>), dim3(nb), dim3(256), 0, c->stream, m);
+        }
         HCHECK(c, hipGetLastError());
     }
     FaceArgs<R> fa;
@@ -167,7 +212,10 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     fa.g = c->g;
     fa.f = c->ctf;
     fa.s = s;
-    hipLaunchKernelGGL((k_faces<R>), dim3(nb), dim3(256), 0, c->stream, fa);
+    {
+        TimedLaunch tl(c, HOWMHD_K_FACES);
+        hipLaunchKernelGGL((k_faces<R>), dim3(nb), dim3(256), 0, c->stream, fa);
+    }
     HCHECK(c, hipGetLastError());
 
     CombineArgs<R> ca;
@@ -184,7 +232,10 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     ca.f = c->ctf;
     ca.s = s;
     ca.ph = c->ph;
-    hipLaunchKernelGGL((k_combine<R>), dim3(nb), dim3(256), 0, c->stream, ca);
+    {
+        TimedLaunch tl(c, HOWMHD_K_COMBINE);
+        hipLaunchKernelGGL((k_combine<R>), dim3(nb), dim3(256), 0, c->stream, ca);
+    }
     HCHECK(c, hipGetLastError());
     return HOWMHD_OK;
 }
@@ -310,20 +361,21 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
 
 int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
     if (!c) return HOWMHD_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 5; ++b) {
-        if (c->qbuf[b]) hipFree(c->qbuf[b]);
-        if (c->fbuf[b]) hipFree(c->fbuf[b]);
+        if (c->qbuf[b]) (void)hipFree(c->qbuf[b]);
+        if (c->fbuf[b]) (void)hipFree(c->fbuf[b]);
     }
     for (int d = 0; d < 3; ++d) {
-        if (c->dF[d]) hipFree(c->dF[d]);
-        if (c->O2[d]) hipFree(c->O2[d]);
-        if (c->O[d]) hipFree(c->O[d]);
+        if (c->dF[d]) (void)hipFree(c->dF[d]);
+        if (c->O2[d]) (void)hipFree(c->O2[d]);
+        if (c->O[d]) (void)hipFree(c->O[d]);
     }
     for (int t = 0; t < 6; ++t)
-        if (c->ct[t]) hipFree(c->ct[t]);
-    if (c->dmax) hipFree(c->dmax);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (c->ct[t]) (void)hipFree(c->ct[t]);
+    if (c->dmax) (void)hipFree(c->dmax);
+    drain_timing(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HOWMHD_OK;
 }
@@ -365,8 +417,11 @@ int howmhd_gpu_tstep(howmhd_gpu_ctx* c, double* dt) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->dmax, 3);
     HCHECK(c, hipGetLastError());
     const unsigned nb = std::min<unsigned>(blocks_for(c->ncell, 256), 2048u);
-    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, c->stream, (const double*)c->qbuf[c->qid[0]], c->g, c->ph,
-                       c->dmax);
+    {
+        TimedLaunch tl(c, HOWMHD_K_CFL);
+        hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, c->stream, (const double*)c->qbuf[c->qid[0]], c->g, c->ph,
+                           c->dmax);
+    }
     HCHECK(c, hipGetLastError());
     unsigned long long h[3];
     HCHECK(c, hipMemcpyAsync(h, c->dmax, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -481,6 +536,23 @@ int howmhd_gpu_diag(howmhd_gpu_ctx* c, howmhd_diag* d) {
                 }
                 d->max_divb_cell = std::max(d->max_divb_cell, std::fabs(dc));
             }
+    return HOWMHD_OK;
+}
+
+int howmhd_gpu_set_timing(howmhd_gpu_ctx* c, int on) {
+    if (!c) return HOWMHD_EINVAL;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    drain_timing(c);
+    for (int k = 0; k < HOWMHD_K_COUNT; ++k) { c->tms[k] = 0.0; c->tcount[k] = 0; }
+    c->timing = (on != 0);
+    return HOWMHD_OK;
+}
+
+int howmhd_gpu_get_timing(howmhd_gpu_ctx* c, howmhd_timing* t) {
+    if (!c || !t) return HOWMHD_EINVAL;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    drain_timing(c);
+    for (int k = 0; k < HOWMHD_K_COUNT; ++k) { t->ms[k] = c->tms[k]; t->launches[k] = c->tcount[k]; }
     return HOWMHD_OK;
 }
 

@@ -47,6 +47,13 @@ class Diag(C.Structure):
     ]
 
 
+KERNELS = ["sweep_x", "sweep_y", "sweep_z", "emf2", "emf", "faces", "combine", "cfl"]
+
+
+class Timing(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int32 * 8)]
+
+
 class Run(C.Structure):
     _fields_ = [
         ("gnx", C.c_int32), ("gny", C.c_int32), ("gnz", C.c_int32),
@@ -100,6 +107,8 @@ GPU_FUNCS = {
     "howmhd_gpu_download_faces": (C.c_int, [C.c_void_p, _DP]),
     "howmhd_gpu_diag": (C.c_int, [C.c_void_p, C.POINTER(Diag)]),
     "howmhd_gpu_synchronize": (C.c_int, [C.c_void_p]),
+    "howmhd_gpu_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "howmhd_gpu_get_timing": (C.c_int, [C.c_void_p, C.POINTER(Timing)]),
     "howmhd_gpu_last_error": (C.c_char_p, [C.c_void_p]),
     "howmhd_gpu_build_kind": (C.c_char_p, []),
 }
@@ -236,6 +245,15 @@ class GpuSolver:
         d = Diag()
         self._check(self.lib.howmhd_gpu_diag(self.h, C.byref(d)), "diag")
         return d
+
+    def set_timing(self, on: bool):
+        self._check(self.lib.howmhd_gpu_set_timing(self.h, int(on)), "set_timing")
+
+    def timing(self) -> dict:
+        """{kernel: (total ms, launches)} of the launches timed since set_timing(True)."""
+        t = Timing()
+        self._check(self.lib.howmhd_gpu_get_timing(self.h, C.byref(t)), "get_timing")
+        return {k: (t.ms[i], t.launches[i]) for i, k in enumerate(KERNELS)}
 
     def close(self):
         if getattr(self, "h", None):

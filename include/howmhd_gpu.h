@@ -67,6 +67,17 @@ typedef struct howmhd_diag {
     double flagged_fraction; /* indicator: flagged interfaces / all, last step   */
 } howmhd_diag;
 
+/* Per-kernel device time of the launches issued while timing was on,
+ * measured with HIP events on the context's stream. */
+enum {
+    HOWMHD_K_SWEEP_X = 0, HOWMHD_K_SWEEP_Y, HOWMHD_K_SWEEP_Z, HOWMHD_K_EMF2, HOWMHD_K_EMF,
+    HOWMHD_K_FACES, HOWMHD_K_COMBINE, HOWMHD_K_CFL, HOWMHD_K_COUNT
+};
+typedef struct howmhd_timing {
+    double ms[HOWMHD_K_COUNT];         /* summed kernel time, milliseconds */
+    int32_t launches[HOWMHD_K_COUNT];  /* number of launches timed        */
+} howmhd_timing;
+
 typedef struct howmhd_gpu_ctx howmhd_gpu_ctx;
 
 /* Allocates q (5 stage buffers), faces (5 stage buffers) and the sweep / CT
@@ -108,6 +119,13 @@ int howmhd_gpu_download_faces(howmhd_gpu_ctx* ctx, double* b);
 
 /* Diagnostics of the stage-0 state.  Synchronises. */
 int howmhd_gpu_diag(howmhd_gpu_ctx* ctx, howmhd_diag* d);
+
+/* Starts (on != 0, counters reset) or stops per-kernel timing.  No
+ * counterpart in the reference, which has no timers (src/ has no clock). */
+int howmhd_gpu_set_timing(howmhd_gpu_ctx* ctx, int on);
+
+/* Resolves the timed launches into *t.  Synchronises. */
+int howmhd_gpu_get_timing(howmhd_gpu_ctx* ctx, howmhd_timing* t);
 
 /* Waits for all work queued on the context's stream. */
 int howmhd_gpu_synchronize(howmhd_gpu_ctx* ctx);

@@ -297,21 +297,58 @@ def test_floors_bound_pressure():
 
 
 # ------------------------------------------------------------ convergence
-def test_convergence_order_alfven_wave():
+def _run_to(s, tend):
+    t = 0.0
+    while t < tend:
+        dt = min(s.tstep(), tend - t)
+        s.ssprk(dt)
+        t += dt
+
+
+def _entropy_wave(n):
+    """Density wave advected along (1,2,2)/3 at speed 1 through uniform p and B
+    in the 3 x 1.5 x 1.5 box: an exact solution returning to itself at t = 1."""
+    L = (3.0, 1.5, 1.5)
+    xs = [(np.arange(n) + 0.5) * L[d] / n for d in range(3)]
+    X, Y, Z = np.meshgrid(*xs, indexing="ij")
+    rho = 1 + 0.2 * np.sin(2 * np.pi * (X + 2 * Y + 2 * Z) / 3)
+    v = np.array([1.0, 2.0, 2.0]) / 3.0
+    B = np.array([0.3, -0.2, 0.4])
+    E = 1.0 / (GAM - 1) + 0.5 * rho * (v @ v) + 0.5 * (B @ B)
+    q0 = np.zeros((8, n + 6, n + 6, n + 6))
+    for m, f in enumerate([rho, rho * v[0], rho * v[1], rho * v[2], B[0] + 0 * X, B[1] + 0 * X, B[2] + 0 * X, E]):
+        q0[m, 3:-3, 3:-3, 3:-3] = f
+    return q0
+
+
+def test_convergence_order_entropy_wave():
+    """WENO5 + SSPRK(5,4) without an EMF: a smooth density wave converges at
+    better than 4th order (12^3 -> 24^3)."""
+    errs = []
+    for n in (12, 24):
+        q0 = _entropy_wave(n)
+        _, p, s, _ = make_solver(q0=q0, nx=n, ny=n, nz=n, xsize=3.0, ysize=1.5, zsize=1.5, cour=0.4)
+        _run_to(s, 1.0)
+        errs.append(np.abs(H.active(s.q0())[0] - H.active(q0)[0]).mean())
+    order = math.log2(errs[0] / errs[1])
+    print("entropy wave L1(rho):", errs, "order", order)
+    assert order > 4.0, (errs, order)
+
+
+def test_convergence_order_alfven_wave_is_ct_limited():
     """Circularly polarised Alfven wave along (1,2,2)/3 (problem cpaw3d), one
-    period: the L1 error of By against the initial state falls at >= 3.5th
-    order from 12^3 to 24^3.  (In 2-D the reference never CT-updates the
-    out-of-plane face field, so a 2-D version of this wave is not usable.)"""
+    period.  The CT flux terms of the reference average B_n v_t over the two
+    cells beside the face (weights (-0, 9, 9, -0)/18, src/ssprk.cpp:979-989),
+    which is 2nd-order accurate, so the face field - and By here - converges
+    at about 2nd order (with a (-1, 9, 9, -1)/16 average the same run gives
+    about 4.6).  The restatement must show that 2nd order, not more and not
+    less.  (In 2-D the reference never CT-updates the out-of-plane face
+    field, so a 2-D version of this wave is not usable.)"""
     errs = []
     for n in (12, 24):
         _, p, s, q0 = make_solver("cpaw3d", nx=n, ny=n, nz=n, xsize=3.0, ysize=1.5, zsize=1.5, cour=0.4)
-        t = 0.0
-        while t < 1.0:
-            dt = min(s.tstep(), 1.0 - t)
-            s.ssprk(dt)
-            t += dt
-        q = s.q0()
-        errs.append(np.abs(H.active(q)[5] - H.active(q0)[5]).mean())
+        _run_to(s, 1.0)
+        errs.append(np.abs(H.active(s.q0())[5] - H.active(q0)[5]).mean())
     order = math.log2(errs[0] / errs[1])
     print("Alfven wave L1(By):", errs, "order", order)
-    assert order > 3.5, (errs, order)
+    assert 1.6 < order < 2.6, (errs, order)
