@@ -62,6 +62,8 @@ def parse_args():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (HBM-counter calibration under rocprofv3)")
     return ap.parse_args()
 
 
@@ -141,6 +143,14 @@ def main():
     el = time.perf_counter() - t0
     zones = n ** 3
     value = zones * args.steps / el
+
+    if args.copy_probe_mib > 0:
+        nel = args.copy_probe_mib * 1024 * 1024 // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # per-kernel device time on the library's stream
     gpu.set_timing(True)

@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "mhd_math.hpp"
 
@@ -56,17 +57,37 @@ struct Phys {
 
 // ------------------------------------------------------------- sweep tiling
 // X sweep: 64 interfaces along x per row, 4 rows (y) per block.
-// Y/Z sweep: 32 x-columns, 8 interfaces along the sweep axis per block.
+// Y/Z sweep: HOWMHD_YZ_LANES x-columns, 256/HOWMHD_YZ_LANES interfaces along the
+// sweep axis per block.
 template <int DIR> struct SweepTile;
 template <> struct SweepTile<0> {
     static constexpr int LANES = 64, ROWS = 4, CELLS = (64 + 5) * 4;
 };
+#ifndef HOWMHD_YZ_LANES
+#define HOWMHD_YZ_LANES 32
+#endif
 template <> struct SweepTile<1> {
-    static constexpr int LANES = 32, ROWS = 8, CELLS = 32 * (8 + 5);
+    static constexpr int LANES = HOWMHD_YZ_LANES, ROWS = 256 / HOWMHD_YZ_LANES, CELLS = LANES * (ROWS + 5);
 };
 template <> struct SweepTile<2> {
-    static constexpr int LANES = 32, ROWS = 8, CELLS = 32 * (8 + 5);
+    static constexpr int LANES = HOWMHD_YZ_LANES, ROWS = 256 / HOWMHD_YZ_LANES, CELLS = LANES * (ROWS + 5);
 };
+
+// Waves whose characteristic projections share one pass over the stencil in
+// LDS, and the minimum number of wavefronts per SIMD the sweep kernel is
+// compiled for (caps its registers at 512 / that number).
+#ifndef HOWMHD_WAVE_GROUP
+#define HOWMHD_WAVE_GROUP 1
+#endif
+#ifndef HOWMHD_SWEEP_WAVES
+#define HOWMHD_SWEEP_WAVES 2
+#endif
+// Stencil cells whose LDS reads the scheduler may overlap; a scheduling
+// barrier after every such batch keeps it from hoisting all 42 reads of a
+// group ahead of the arithmetic (which spills registers).
+#ifndef HOWMHD_PROJ_BATCH
+#define HOWMHD_PROJ_BATCH 3
+#endif
 
 template <typename R>
 struct SweepArgs {
@@ -101,36 +122,104 @@ __device__ __forceinline__ R wave_speed(R vn, R lf, R la, R ls) {
     else return vn + lf;
 }
 
-// One wave of the characteristic flux: project the 6-cell stencil on L row M,
-// WENO5 with LLF splitting, add R column M times the wave flux to dF
-// (src/ssprk.cpp:847-943, waves added in order 0..6).
-template <int M, typename R>
-__device__ __forceinline__ void wave_flux(R gam, const Eig<R>& e, const R (&U)[6][7], const R (&F)[6][7],
-                                          const R (&spd)[6][4], R (&dF)[7]) {
-    R amx = R(0);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) amx = rmax(amx, rabs(wave_speed<M>(spd[k][0], spd[k][1], spd[k][2], spd[k][3])));
-    R L[7];
-    l_row<M>(gam, e, L);
-    R Fsk[6], qsk[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        Fsk[k] = L[0] * F[k][0] + L[1] * F[k][1] + L[2] * F[k][2] + L[3] * F[k][3] + L[4] * F[k][4] +
-                 L[5] * F[k][5] + L[6] * F[k][6];
-        qsk[k] = L[0] * U[k][0] + L[1] * U[k][1] + L[2] * U[k][2] + L[3] * U[k][3] + L[4] * U[k][4] +
-                 L[5] * U[k][5] + L[6] * U[k][6];
+template <int B, int E, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, E>(f);
     }
-    const R Fs = weno_wave<R>(Fsk, qsk, amx);
-    R Rc[7];
-    r_col<M>(gam, e, Rc);
-#pragma unroll
-    for (int r = 0; r < 7; ++r) dF[r] += Rc[r] * Fs;
+}
+
+// (U_j, F_j) of one cell side by side, so one 128-bit LDS read serves both.
+template <typename R>
+struct alignas(2 * sizeof(R)) UF {
+    R u, f;
+};
+
+// Per-cell fields besides U and F, in LDS.
+enum OtherField { O_VN = 0, O_LF, O_LA, O_LS, O_VT1, O_VT2, O_BN, O_P, O_H, O_COUNT };
+
+template <typename R>
+__device__ __forceinline__ void opq(R& x) { asm volatile("" : "+v"(x)); }
+
+// Makes x opaque to the optimiser and, through the input operand, ordered
+// after `after` (no instruction emitted).
+template <typename R>
+__device__ __forceinline__ void opq_after(R& x, R after) { asm volatile("" : "+v"(x) : "v"(after)); }
+
+// The scalars the R columns use, made opaque after the wave flux exists: the
+// column is then built after the WENO step instead of being kept live across it.
+template <typename R>
+__device__ __forceinline__ void opaque_after(Eig<R>& o, R a) {
+    opq_after(o.vx, a); opq_after(o.vy, a); opq_after(o.vz, a); opq_after(o.vv2, a);
+    opq_after(o.cs, a); opq_after(o.cs2, a); opq_after(o.lf, a); opq_after(o.ls, a);
+    opq_after(o.af, a); opq_after(o.as, a); opq_after(o.bty, a); opq_after(o.btz, a);
+    opq_after(o.sgnBx, a); opq_after(o.invsqrtrho, a); opq_after(o.sgnBt, a);
+}
+
+// Makes every scalar of e opaque to the optimiser (no instruction emitted).
+template <typename R>
+__device__ __forceinline__ void opaque(Eig<R>& o) {
+    opq(o.vx); opq(o.vy); opq(o.vz); opq(o.By); opq(o.Bz); opq(o.vv2); opq(o.cs); opq(o.cs2);
+    opq(o.lf); opq(o.ls); opq(o.af); opq(o.as); opq(o.bty); opq(o.btz); opq(o.sgnBx);
+    opq(o.sqrtrho); opq(o.invsqrtrho); opq(o.sgnBt); opq(o.nrm_half); opq(o.nrm_ent);
+}
+
+// a[m] for a wave index known only at run time, without indexing a private
+// array (which would live in scratch memory).
+template <typename R>
+struct Seven {
+    R v0, v1, v2, v3, v4, v5, v6;
+};
+template <typename R>
+__device__ __forceinline__ R pick7(const Seven<R>& a, int m) {
+    R v;
+    switch (m) {  // scalar branch; the asm keeps it from becoming an indexed load
+    case 0: asm volatile(""); v = a.v0; break;
+    case 1: asm volatile(""); v = a.v1; break;
+    case 2: asm volatile(""); v = a.v2; break;
+    case 3: asm volatile(""); v = a.v3; break;
+    case 4: asm volatile(""); v = a.v4; break;
+    case 5: asm volatile(""); v = a.v5; break;
+    default: asm volatile(""); v = a.v6; break;
+    }
+    return v;
+}
+
+// Row m of L and column m of R for a wave index known only at run time (the
+// same for the whole wavefront, so the switch is a scalar branch).  The empty
+// volatile asm in each case keeps the compiler from evaluating all seven
+// cases and selecting one.
+template <typename R>
+__device__ __forceinline__ void l_row_dyn(int m, R gam, const Eig<R>& e, R* L) {
+    switch (m) {
+    case 0: asm volatile(""); l_row<0>(gam, e, L); break;
+    case 1: asm volatile(""); l_row<1>(gam, e, L); break;
+    case 2: asm volatile(""); l_row<2>(gam, e, L); break;
+    case 3: asm volatile(""); l_row<3>(gam, e, L); break;
+    case 4: asm volatile(""); l_row<4>(gam, e, L); break;
+    case 5: asm volatile(""); l_row<5>(gam, e, L); break;
+    default: asm volatile(""); l_row<6>(gam, e, L); break;
+    }
+}
+template <typename R>
+__device__ __forceinline__ void r_col_dyn(int m, R gam, const Eig<R>& e, R* Rc) {
+    switch (m) {
+    case 0: asm volatile(""); r_col<0>(gam, e, Rc); break;
+    case 1: asm volatile(""); r_col<1>(gam, e, Rc); break;
+    case 2: asm volatile(""); r_col<2>(gam, e, Rc); break;
+    case 3: asm volatile(""); r_col<3>(gam, e, Rc); break;
+    case 4: asm volatile(""); r_col<4>(gam, e, Rc); break;
+    case 5: asm volatile(""); r_col<5>(gam, e, Rc); break;
+    default: asm volatile(""); r_col<6>(gam, e, Rc); break;
+    }
 }
 
 template <typename R, int DIR>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs<R> A) {
+__global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> A) {
     using T = SweepTile<DIR>;
-    __shared__ R lds[C_NFIELD][T::CELLS];
+    __shared__ UF<R> luf[7][T::CELLS];
+    __shared__ R lo[O_COUNT][T::CELLS];
     const Geom& g = A.g;
     const int nd = g.n[DIR];
     const int per = g.per[DIR];
@@ -178,7 +267,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<R> A) {
         R out[C_NFIELD];
         cell_values<R>(qs, A.ph.gam, A.ph.rhomin, A.ph.pgmin, out);
 #pragma unroll
-        for (int f = 0; f < C_NFIELD; ++f) lds[f][s] = out[f];
+        for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
+#pragma unroll
+        for (int f = 0; f < O_COUNT; ++f) lo[f][s] = out[C_VN + f];
     }
     __syncthreads();
 
@@ -200,36 +291,86 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<R> A) {
         else return (row + k) * T::LANES + lane;
     };
 
-    R U[6][7], F[6][7], spd[6][4];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const int s = slot(k);
-#pragma unroll
-        for (int j = 0; j < 7; ++j) {
-            U[k][j] = lds[C_RHO + j][s];
-            F[k][j] = lds[C_F0 + j][s];
-        }
-        spd[k][0] = lds[C_VN][s]; spd[k][1] = lds[C_LF][s]; spd[k][2] = lds[C_LA][s]; spd[k][3] = lds[C_LS][s];
-    }
     // interface state from cells k = 2, 3: (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H)
     R u0[9], u1[9];
     {
         const int s0 = slot(2), s1 = slot(3);
-        const int fl[9] = {C_RHO, C_VN, C_VT1, C_VT2, C_BN, C_BT1, C_BT2, C_P, C_H};
-#pragma unroll
-        for (int v = 0; v < 9; ++v) { u0[v] = lds[fl[v]][s0]; u1[v] = lds[fl[v]][s1]; }
+        u0[0] = luf[0][s0].u; u0[1] = lo[O_VN][s0]; u0[2] = lo[O_VT1][s0]; u0[3] = lo[O_VT2][s0];
+        u0[4] = lo[O_BN][s0]; u0[5] = luf[4][s0].u; u0[6] = luf[5][s0].u; u0[7] = lo[O_P][s0]; u0[8] = lo[O_H][s0];
+        u1[0] = luf[0][s1].u; u1[1] = lo[O_VN][s1]; u1[2] = lo[O_VT1][s1]; u1[3] = lo[O_VT2][s1];
+        u1[4] = lo[O_BN][s1]; u1[5] = luf[4][s1].u; u1[6] = luf[5][s1].u; u1[7] = lo[O_P][s1]; u1[8] = lo[O_H][s1];
     }
     const R gam = R(A.ph.gam);
     Eig<R> e;
     eig_scalars<R>(gam, u0, u1, e);
+
+    // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
+    Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int s = slot(k);
+        const R vn = lo[O_VN][s], lf = lo[O_LF][s], la = lo[O_LA][s], ls = lo[O_LS][s];
+        amx.v0 = rmax(amx.v0, rabs(vn - lf));
+        amx.v1 = rmax(amx.v1, rabs(vn - la));
+        amx.v2 = rmax(amx.v2, rabs(vn - ls));
+        amx.v3 = rmax(amx.v3, rabs(vn));
+        amx.v4 = rmax(amx.v4, rabs(vn + ls));
+        amx.v5 = rmax(amx.v5, rabs(vn + la));
+        amx.v6 = rmax(amx.v6, rabs(vn + lf));
+    }
+
+    // Waves are taken in pairs (0,1), (2,3), (4,5), (6): both waves of a pair
+    // are projected in one pass over the stencil in LDS, and dF still gets
+    // the waves added in order 0..6.
     R dF[7] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-    wave_flux<0, R>(gam, e, U, F, spd, dF);
-    wave_flux<1, R>(gam, e, U, F, spd, dF);
-    wave_flux<2, R>(gam, e, U, F, spd, dF);
-    wave_flux<3, R>(gam, e, U, F, spd, dF);
-    wave_flux<4, R>(gam, e, U, F, spd, dF);
-    wave_flux<5, R>(gam, e, U, F, spd, dF);
-    wave_flux<6, R>(gam, e, U, F, spd, dF);
+    const R amxv[7] = {amx.v0, amx.v1, amx.v2, amx.v3, amx.v4, amx.v5, amx.v6};
+    sfor<0, 4>([&](auto pc) {
+        constexpr int m0 = 2 * decltype(pc)::value;
+        constexpr bool two = (m0 + 1 < 7);
+        constexpr int m1 = two ? m0 + 1 : m0;
+        // the stencil is read from LDS again for every pair rather than kept
+        // live in registers
+        asm volatile("" ::: "memory");
+        // the scalars become opaque here, so the compiler builds this pair's
+        // rows and columns here instead of all seven up front (98 live values)
+        opaque(e);
+        R La[7], Lb[7];
+        l_row<m0>(gam, e, La);
+        if constexpr (two) l_row<m1>(gam, e, Lb);
+        R Fa[6], qa[6], Fb[6], qb[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int s = slot(k);
+            R U[7], F[7];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const UF<R> v = luf[j][s];
+                U[j] = v.u;
+                F[j] = v.f;
+            }
+            Fa[k] = La[0] * F[0] + La[1] * F[1] + La[2] * F[2] + La[3] * F[3] + La[4] * F[4] + La[5] * F[5] + La[6] * F[6];
+            qa[k] = La[0] * U[0] + La[1] * U[1] + La[2] * U[2] + La[3] * U[3] + La[4] * U[4] + La[5] * U[5] + La[6] * U[6];
+            if constexpr (two) {
+                Fb[k] = Lb[0] * F[0] + Lb[1] * F[1] + Lb[2] * F[2] + Lb[3] * F[3] + Lb[4] * F[4] + Lb[5] * F[5] + Lb[6] * F[6];
+                qb[k] = Lb[0] * U[0] + Lb[1] * U[1] + Lb[2] * U[2] + Lb[3] * U[3] + Lb[4] * U[4] + Lb[5] * U[5] + Lb[6] * U[6];
+            }
+            if ((k + 1) % HOWMHD_PROJ_BATCH == 0) asm volatile("" ::: "memory");
+        }
+        const R Fsa = weno_wave<R>(Fa, qa, amxv[m0]);
+        R Fsb = R(0);
+        if constexpr (two) Fsb = weno_wave<R>(Fb, qb, amxv[m1]);
+        opaque_after(e, Fsa);
+        if constexpr (two) opaque_after(e, Fsb);
+        R Rc[7];
+        r_col<m0>(gam, e, Rc);
+#pragma unroll
+        for (int r = 0; r < 7; ++r) dF[r] += Rc[r] * Fsa;
+        if constexpr (two) {
+            r_col<m1>(gam, e, Rc);
+#pragma unroll
+            for (int r = 0; r < 7; ++r) dF[r] += Rc[r] * Fsb;
+        }
+    });
 
 #pragma unroll
     for (int c = 0; c < 7; ++c) A.dF[dF_index<DIR>(g, c, t, x, y, z)] = dF[c];
@@ -241,9 +382,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<R> A) {
 #pragma unroll
         for (int k = 1; k <= 4; ++k) {
             const int s = slot(k);
-            const R bn = lds[C_BN][s];
-            bnv1[k - 1] = bn * lds[C_VT1][s];
-            bnv2[k - 1] = bn * lds[C_VT2][s];
+            const R bn = lo[O_BN][s];
+            bnv1[k - 1] = bn * lo[O_VT1][s];
+            bnv2[k - 1] = bn * lo[O_VT2][s];
         }
         const R ftmp1 = div_c<R>(R(-0.0) * bnv1[0] + R(9) * bnv1[1] + R(9) * bnv1[2] - R(0) * bnv1[3], R(18),
                                  R(1.0 / 18.0));

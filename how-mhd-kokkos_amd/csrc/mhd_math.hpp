@@ -80,9 +80,16 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const double Bx = qs[4], By = qs[5], Bz = qs[6];
     double EE = qs[7];
     double rho = DD;
+#if HOWMHD_EXACT
     const double vx = Mx / rho;
     const double vy = My / rho;
     const double vz = Mz / rho;
+#else
+    double irho = 1.0 / rho;
+    const double vx = Mx * irho;
+    const double vy = My * irho;
+    const double vz = Mz * irho;
+#endif
     const double vv2 = vx * vx + vy * vy + vz * vz;
     const double BB2 = Bx * Bx + By * By + Bz * Bz;
     double pg = (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2));
@@ -92,8 +99,15 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
         pg = fmax(pgmin, pg);
         EE = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
         Dq = rho;
+#if !HOWMHD_EXACT
+        irho = 1.0 / rho;
+#endif
     }
+#if HOWMHD_EXACT
     const double HH = (EE + pg) / rho;
+#else
+    const double HH = (EE + pg) * irho;
+#endif
 
     // eigenst centre, in R
     const R rrho = R(rho), rvx = R(vx), rvy = R(vy), rvz = R(vz);
@@ -103,9 +117,16 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const R rBB2 = rBx * rBx + rBy * rBy + rBz * rBz;
     const R pt = rpg + R(0.5) * rBB2;
     const R vdB = rvx * rBx + rvy * rBy + rvz * rBz;
+#if HOWMHD_EXACT
     const R bbn2 = rBB2 / rrho;
     const R bnx2 = rBx * rBx / rrho;
     const R cs2 = rmax(R(0), rgam * rabs(rpg / rrho));
+#else
+    const R rirho = R(irho);
+    const R bbn2 = rBB2 * rirho;
+    const R bnx2 = rBx * rBx * rirho;
+    const R cs2 = rmax(R(0), rgam * rabs(rpg * rirho));
+#endif
     const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
     const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
     const R la = rsqrt_(rmax(R(0), bnx2));
@@ -131,6 +152,7 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
 template <typename R>
 struct Eig {
     R vx, vy, vz, By, Bz, vv2, cs, cs2, lf, ls, af, as, bty, btz, sgnBx, sqrtrho, invsqrtrho, sgnBt;
+    R nrm_half, nrm_ent;  // product build: 0.5/cs2 and (gam-1)/cs2, the row normalisers
     bool slow_flip;  // cs >= la: waves 2 and 4 take the sign of Bt; else waves 0 and 6
 };
 
@@ -147,8 +169,14 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     const R HH = R(0.5) * (u0[8] + u1[8]);
     const R vv2 = vx * vx + vy * vy + vz * vz;
     const R BB2 = Bx * Bx + By * By + Bz * Bz;
+#if HOWMHD_EXACT
     const R bbn2 = BB2 / rho;
     const R bnx2 = Bx * Bx / rho;
+#else
+    const R irho = R(1) / rho;
+    const R bbn2 = BB2 * irho;
+    const R bnx2 = Bx * Bx * irho;
+#endif
     const R cs2 = (gam - R(1)) * (HH - R(0.5) * (vv2 + bbn2));
     const R cs = rsqrt_(rmax(R(0), cs2));
     const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
@@ -158,16 +186,28 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     const R Bt2 = By * By + Bz * Bz;
     R bty, btz;
     if (Bt2 >= R(1.0e-30)) {
+#if HOWMHD_EXACT
         bty = By / rsqrt_(Bt2);
         btz = Bz / rsqrt_(Bt2);
+#else
+        const R rb = R(1) / rsqrt_(Bt2);
+        bty = By * rb;
+        btz = Bz * rb;
+#endif
     } else {
         bty = R(1) / rsqrt_(R(2));
         btz = R(1) / rsqrt_(R(2));
     }
     R af, as;
     if ((lf * lf - ls * ls) >= R(1.0e-30)) {
+#if HOWMHD_EXACT
         af = rsqrt_(rmax(R(0), cs2 - ls * ls)) / rsqrt_(lf * lf - ls * ls);
         as = rsqrt_(rmax(R(0), lf * lf - cs2)) / rsqrt_(lf * lf - ls * ls);
+#else
+        const R rd = R(1) / rsqrt_(lf * lf - ls * ls);
+        af = rsqrt_(rmax(R(0), cs2 - ls * ls)) * rd;
+        as = rsqrt_(rmax(R(0), lf * lf - cs2)) * rd;
+#endif
     } else {
         af = R(1);
         as = R(1);
@@ -180,6 +220,14 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     e.invsqrtrho = R(1) / sqrtrho;
     e.sgnBt = (By != R(0)) ? sign1(By) : sign1(Bz);
     e.slow_flip = (cs >= la);
+#if !HOWMHD_EXACT
+    const R ics2 = R(1) / cs2;
+    e.nrm_half = R(0.5) * ics2;
+    e.nrm_ent = (gam - R(1)) * ics2;
+#else
+    e.nrm_half = R(0);
+    e.nrm_ent = R(0);
+#endif
 }
 
 // Row m of L after normalisation and sign flip (src/ssprk.cpp:662-726,
@@ -251,9 +299,15 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
     // normalisation, src/ssprk.cpp:718-726
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
+#if HOWMHD_EXACT
         if constexpr (M == 1 || M == 5) L[c] = R(0.5) * L[c];
         else if constexpr (M == 3) L[c] = -gam0 * L[c] / cs2;
         else L[c] = R(0.5) * L[c] / cs2;
+#else
+        if constexpr (M == 1 || M == 5) L[c] = R(0.5) * L[c];
+        else if constexpr (M == 3) L[c] = L[c] * e.nrm_ent;
+        else L[c] = L[c] * e.nrm_half;
+#endif
     }
     // sign of the transverse field, src/ssprk.cpp:796-806
     if constexpr (M == 2 || M == 4) {

@@ -1,0 +1,250 @@
+"""Host shell and C-ABI surface (CPU only).
+
+The parser, problem registry and writers restate src/parameters.cpp,
+src/problem.cpp, src/problems/*.cpp and src/output.cpp; these tests check their
+documented behaviour on small inputs, that the libraries load, and that every
+function declared in include/*.h is exported.
+"""
+import math
+import re
+import struct
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import howmhd as H
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions(header: Path):
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(howmhd_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,libs", [
+    ("howmhd_gpu.h", ["libhowmhd_gpu.so", "libhowmhd_gpu_exact.so"]),
+    ("howmhd_host.h", ["libhowmhd_host.so"]),
+])
+def test_every_declared_symbol_is_exported(header, libs):
+    import ctypes
+    names = declared_functions(ROOT / "include" / header)
+    assert names
+    for lib in libs:
+        dll = ctypes.CDLL(str(H.LIB_DIR / lib))
+        missing = [n for n in names if not hasattr(dll, n)]
+        assert not missing, (lib, missing)
+
+
+def test_build_kinds():
+    assert H.build_kind(exact=False) == "fp-fast"
+    assert H.build_kind(exact=True) == "fp-exact"
+
+
+def test_create_fails_cleanly_without_gpu_or_with_bad_params():
+    run = H.parse(H.make_input(problem="uniform", nx=2, ny=4, nz=4, nt=1, tend=1))
+    p = H.gpu_params(run)
+    with pytest.raises(H.HowmhdError, match="n == 1 or n >= 3"):
+        H.GpuSolver(p)
+
+
+# ----------------------------------------------------------------- parser
+def test_parser_defaults_and_keys():
+    text = """# comment line
+problem orszag_tang
+nx 16 ny 8
+nz 1   # trailing comment is a token starting with #
+nt 2
+tend 0.5
+gam 1.4
+cour 0.8
+rhomin 1e-8
+x1bc open
+output_format vts
+"""
+    r = H.parse(text)
+    assert (r.gnx, r.gny, r.gnz) == (16, 8, 1)
+    assert (r.nx, r.ny, r.nz) == (16, 8, 1)
+    assert r.problem == b"orszag_tang"
+    assert r.gam == 1.4 and r.cour == 0.8 and r.tend == 0.5 and r.nt == 2
+    assert r.rhomin == 1e-8 and r.pgmin == 1e-12          # default pgmin
+    assert r.x1bc == b"open" and r.x2bc == b"periodic"
+    assert r.output_format == b"vts"
+    assert r.dx == 1.0 / 16 and r.dy == 1.0 / 8 and r.dz == 1.0
+    assert r.precision == b"fp64" and r.indicator == b"off"
+
+
+def test_parser_gnx_alias_and_unknown_key(capfd):
+    r = H.parse("gnx 8 gny 4 gnz 2 nt 1 tend 1 bogus 3 4 5\nxsize 2\n")
+    assert (r.nx, r.ny, r.nz) == (8, 4, 2)
+    assert r.xsize == 2.0 and r.dx == 0.25
+    assert "Warning: unknown parameter bogus" in capfd.readouterr().err
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("nx 0 ny 1 nz 1 nt 1 tend 1", "global grid size must be positive"),
+    ("nx 4 ny 4 nz 4 nt 0 tend 1", "nt must be positive"),
+    ("nx 4 ny 4 nz 4 nt 1 tend 0", "tend must be positive"),
+])
+def test_parser_errors(text, msg):
+    with pytest.raises(H.HowmhdError, match=msg):
+        H.parse(text)
+
+
+def test_unknown_bc_and_problem():
+    r = H.parse("nx 4 ny 4 nz 4 nt 1 tend 1 x2bc reflecting problem nope")
+    with pytest.raises(H.HowmhdError, match="unknown x2bc = reflecting"):
+        H.gpu_params(r)
+    with pytest.raises(H.HowmhdError, match="unknown problem = nope") as e:
+        H.init_problem(r)
+    for name in ("uniform", "orszag_tang", "brio_wu", "mhd_rotor", "blast3d", "mpi_smooth3d"):
+        assert name in str(e.value)
+
+
+# --------------------------------------------------------------- problems
+def test_registry_names():
+    names = H.problem_names()
+    for name in ("uniform", "orszag_tang", "brio_wu", "mhd_rotor", "blast3d", "mpi_smooth3d",
+                 "cpaw3d", "mhd_blast3d", "turbulence3d"):
+        assert name in names
+
+
+def test_orszag_tang_closed_form():
+    r = H.parse(H.make_input(problem="orszag_tang", nx=8, ny=8, nz=1, nt=1, tend=1))
+    q = H.init_problem(r)
+    pi = math.pi
+    i, j = 3 + 2, 3 + 5
+    x, y = (2 + 0.5) / 8, (5 + 0.5) / 8
+    rho = 25 / (36 * pi)
+    assert q[0, i, j, 3] == rho
+    assert q[1, i, j, 3] == rho * -math.sin(2 * pi * y)
+    assert q[4, i, j, 3] == -math.sin(2 * pi * y) / math.sqrt(4 * pi)
+    assert q[5, i, j, 3] == math.sin(4 * pi * x) / math.sqrt(4 * pi)
+    vv2 = math.sin(2 * pi * y) ** 2 + math.sin(2 * pi * x) ** 2
+    BB2 = q[4, i, j, 3] ** 2 + q[5, i, j, 3] ** 2
+    E = 5 / (12 * pi) / (5 / 3 - 1) + 0.5 * rho * vv2 + 0.5 * BB2
+    assert abs(q[7, i, j, 3] - E) < 1e-15
+    # ghosts untouched
+    assert np.all(q[:, :3] == 0) and np.all(q[:, :, :, :3] == 0)
+
+
+def test_blast_variants():
+    for prob, pin, pout, B in (("blast3d", 100.0, 1.0, (0.1, 0.0, 0.0)),
+                               ("mhd_blast3d", 10.0, 0.1, (1 / math.sqrt(2), 1 / math.sqrt(2), 0.0))):
+        r = H.parse(H.make_input(problem=prob, nx=20, ny=20, nz=20, nt=1, tend=1))
+        q = H.active(H.init_problem(r))
+        assert np.all(q[0] == 1.0)
+        BB2 = sum(b * b for b in B)
+        p = (2.0 / 3.0) * (q[7] - 0.5 * BB2)
+        assert set(np.round(np.unique(p), 12)) == {round(pout, 12), round(pin, 12)}
+        c = q[7, 10, 10, 10]   # cell centre at (10.5/20) is inside r < 0.1
+        assert abs((2.0 / 3.0) * (c - 0.5 * BB2) - pin) < 1e-12
+        assert np.allclose(q[4], B[0]) and np.allclose(q[5], B[1])
+
+
+def test_cpaw3d_is_divergence_free_and_alfvenic():
+    r = H.parse(H.make_input(problem="cpaw3d", nx=12, ny=12, nz=12, xsize=3, ysize=1.5, zsize=1.5, nt=1, tend=1))
+    q = H.active(H.init_problem(r))
+    B = q[4:7]
+    v = q[1:4] / q[0]
+    e1 = np.array([1, 2, 2]) / 3
+    bpar = np.einsum("d...,d->...", B, e1)
+    assert np.allclose(bpar, 1.0)
+    assert np.allclose(v + (B - e1[:, None, None, None]), 0.0)      # v_perp = -B_perp
+    perp = B - e1[:, None, None, None]
+    assert np.allclose(np.sqrt((perp ** 2).sum(axis=0)), 0.1)
+
+
+def test_turbulence3d_seeded_and_normalised():
+    kw = dict(problem="turbulence3d", nx=16, ny=16, nz=16, nt=1, tend=1)
+    a = H.active(H.init_problem(H.parse(H.make_input(**kw, seed=7))))
+    b = H.active(H.init_problem(H.parse(H.make_input(**kw, seed=7))))
+    c = H.active(H.init_problem(H.parse(H.make_input(**kw, seed=8))))
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    v = a[1:4] / a[0]
+    assert abs(np.sqrt((v ** 2).sum(axis=0).mean()) - 1.0) < 1e-12
+    # solenoidal: spectral divergence is zero
+    vk = [np.fft.fftn(v[d]) for d in range(3)]
+    k = np.fft.fftfreq(16) * 16
+    KX, KY, KZ = np.meshgrid(k, k, k, indexing="ij")
+    div = KX * vk[0] + KY * vk[1] + KZ * vk[2]
+    assert np.abs(div).max() < 1e-9 * np.abs(vk[0]).max()
+    p = (5 / 3 - 1) * (a[7] - 0.5 * (a[1] ** 2 + a[2] ** 2 + a[3] ** 2) / a[0] - 0.5 * a[6] ** 2)
+    assert np.allclose(p, 0.6)                       # Mach 1 at v_rms = 1
+    assert np.allclose(a[6], math.sqrt(2 * 0.6))     # beta = 2p/B^2 = 1
+
+
+# ---------------------------------------------------------------- writers
+def _state():
+    r = H.parse(H.make_input(problem="mpi_smooth3d", nx=5, ny=4, nz=3, nt=1, tend=1, output_format="vts"))
+    q = H.init_problem(r)
+    # periodic ghosts, as bound(q,0) leaves them
+    for ax in (1, 2, 3):
+        n = q.shape[ax] - 6
+        idx = [(c - 3) % n + 3 for c in range(n + 6)]
+        q = np.take(q, idx, axis=ax)
+    return r, np.ascontiguousarray(q)
+
+
+def test_vts_writer_layout(tmp_path):
+    r, q = _state()
+    H.write_output(r, q, 7, 0.125, str(tmp_path))
+    f = tmp_path / "output" / "dump000007_rank000000.vts"
+    raw = f.read_bytes()
+    head, _, tail = raw.partition(b'<AppendedData encoding="raw">\n_')
+    text = head.decode()
+    assert 'WholeExtent="0 4 0 3 0 2"' in text and 'header_type="UInt64"' in text
+    assert "1.250000000000000e-01" in text
+    names = re.findall(r'Name="(\w+)" NumberOfComponents="(\d)" format="appended" offset="(\d+)"', text)
+    assert [n for n, _, _ in names] == ["rho", "pressure", "energy", "v2", "B2", "divB", "velocity",
+                                        "magnetic_field", "momentum"]
+    npts = 5 * 4 * 3
+    a = H.active(q)
+
+    def block(off, ncomp):
+        nbytes = struct.unpack_from("<Q", tail, off)[0]
+        assert nbytes == 8 * npts * ncomp
+        return np.frombuffer(tail, dtype="<f8", count=npts * ncomp, offset=off + 8)
+
+    offs = {n: (int(o), int(c)) for n, c, o in names}
+    rho = block(*offs["rho"]).reshape(3, 4, 5)          # i fastest
+    assert np.array_equal(rho, a[0].transpose(2, 1, 0))
+    mom = block(*offs["momentum"]).reshape(3, 4, 5, 3)
+    assert np.array_equal(mom[..., 1], a[2].transpose(2, 1, 0))
+    pts_off = int(re.search(r'<Points>\s*<DataArray type="Float64" NumberOfComponents="3" format="appended" '
+                            r'offset="(\d+)"', text).group(1))
+    pts = block(pts_off, 3).reshape(3, 4, 5, 3)
+    assert pts[0, 0, 1, 0] == 1.5 * r.dx and pts[2, 3, 4, 2] == 2.5 * r.dz
+    assert raw.endswith(b"\n  </AppendedData>\n</VTKFile>\n")
+
+
+def test_dat_writer_columns(tmp_path):
+    r, q = _state()
+    r.output_format = b"dat"
+    H.write_output(r, q, 3, 0.5, str(tmp_path))
+    lines = (tmp_path / "output" / "dump000003_rank000000.dat").read_text().splitlines()
+    assert lines[0] == "# step = 3" and lines[1].startswith("# time = 5.000000000000000e-01")
+    assert lines[6].startswith("# columns: ig jg kg il jl kl x y z rho")
+    rows = np.array([[float(v) for v in ln.split()] for ln in lines[7:]])
+    assert rows.shape == (60, 22)
+    a = H.active(q)
+    # second row: ig=1, jg=0, kg=0
+    assert rows[1, 0] == 1 and rows[1, 9] == float(f"{a[0, 1, 0, 0]:.15e}")
+    # periodic centred divB of the cell-centred field
+    i, j, k = 2, 1, 1
+    dv = ((a[4, 3, j, k] - a[4, 1, j, k]) / (2 * r.dx) + (a[5, i, 2, k] - a[5, i, 0, k]) / (2 * r.dy)
+          + (a[6, i, j, 2] - a[6, i, j, 0]) / (2 * r.dz))
+    row = rows[(k * 4 + j) * 5 + i]
+    assert abs(row[21] - dv) <= 1e-12 * max(1.0, abs(dv))
+
+
+def test_output_none_and_unknown(tmp_path):
+    r, q = _state()
+    r.output_format = b"none"
+    H.write_output(r, q, 0, 0.0, str(tmp_path))
+    assert not (tmp_path / "output").exists()
+    r.output_format = b"hdf5"
+    with pytest.raises(H.HowmhdError, match="unknown output_format = hdf5"):
+        H.write_output(r, q, 0, 0.0, str(tmp_path))
