@@ -62,6 +62,8 @@ def parse_args():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP32 form leg")
+    ap.add_argument("--err-steps", type=int, default=10)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM-counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -108,6 +110,62 @@ def cpu_baseline(n: int, steps: int):
         "kind": "port",
         "sample": f"mhd_blast3d {n}^3, {steps} step(s) (tstep + SSPRK(5,4)) of the C restatement of the "
                   f"reference time step, OpenMP, {threads} threads; {el:.2f} s",
+    }
+
+
+def fp32_form(H, torch, run, args, fp64_value):
+    """The precision form (FP32 sweeps and CT EMFs, FP64 stage sums, E->p,
+    floors and face-B): zone-cycles/s on the same workload, its speedup over the
+    plain FP64 path, and its relative L1 error against the FP64 path after
+    --err-steps steps driven with the FP64 path's dt sequence."""
+    import numpy as np
+
+    n = args.size
+    q0 = H.init_problem(run)
+    p64 = H.gpu_params(run)
+    p32 = H.gpu_params(run)
+    p32.precision = H.PREC["fp32"]
+    g32 = H.GpuSolver(p32)
+    g32.upload_q0(q0)
+    g32.init_faces()
+    for _ in range(args.warmup):
+        g32.ssprk(g32.tstep())
+    g32.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g32.ssprk(g32.tstep())
+    g32.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = n ** 3 * args.steps / el
+    # error against the FP64 path, same dt sequence, from the same q(0)
+    g64 = H.GpuSolver(p64)
+    for s in (g32, g64):
+        s.upload_q0(q0)
+        s.init_faces()
+    del q0
+    for _ in range(args.err_steps):
+        dt = g64.tstep()
+        g64.ssprk(dt)
+        g32.ssprk(dt)
+    a32, a64 = g32.q0(), g64.q0()
+    l1 = H.rel_l1(a32, a64)
+    emag = lambda a: 0.5 * float((H.active(a)[4:7] ** 2).sum())
+    d32 = g32.diag()
+    g32.close()
+    g64.close()
+    return {
+        "value": round(value, 1),
+        "unit": "zone-cycles/s",
+        "ms_per_step": round(el / args.steps * 1e3, 3),
+        "speedup_vs_fp64": round(value / fp64_value, 3),
+        "rel_l1_vs_fp64_after_steps": args.err_steps,
+        "rel_l1_rho": float(l1[0]),
+        "rel_l1_E": float(l1[7]),
+        "rel_l1_max": float(np.max(l1)),
+        "magnetic_energy_rel_err": abs(emag(a32) - emag(a64)) / emag(a64),
+        "divb_ct_max": d32.max_divb_ct,
     }
 
 
@@ -216,6 +274,8 @@ def main():
         "whole_step": whole,
         "divb_ct_max": d.max_divb_ct,
     }
+    if not args.no_forms and args.precision == "fp64":
+        out["forms"] = {"fp32": fp32_form(H, torch, run, args, value)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_size, args.cpu_steps)
     print(json.dumps(out), flush=True)
