@@ -7,6 +7,7 @@
 // flux arrays and the six EMF arrays.  All work is queued on the context's
 // stream; tstep and downloads synchronise.
 #include "howmhd_gpu.h"
+#include "howmhd_mlp.h"
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -63,6 +64,11 @@ struct howmhd_gpu_ctx {
     void* O2[3] = {};
     void* O[3] = {};
     unsigned long long* dmax = nullptr;
+    // learned indicator: device weights, routing mode (0 off, 1 on, 2 on and
+    // counted), counter slots
+    MlpDev* mlp = nullptr;
+    int ind_mode = 0;
+    unsigned long long* istats = nullptr;
     bool faces_ready = false;
     bool timing = false;
     struct Timed {
@@ -123,35 +129,65 @@ int fail(howmhd_gpu_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
+template <typename R, int MODE>
+void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
+    const Geom& g = c->g;
+    const int nif = g.per[d] ? g.n[d] : g.n[d] + 1;
+    if (d == 0) {
+        dim3 grid(blocks_for(nif, SweepTile<0>::LANES), blocks_for(g.n[1], SweepTile<0>::ROWS), g.n[2]);
+        hipLaunchKernelGGL((k_sweep<R, 0, MODE>), grid, dim3(256), 0, c->stream, a);
+    } else if (d == 1) {
+        dim3 grid(blocks_for(g.n[0], SweepTile<1>::LANES), blocks_for(nif, SweepTile<1>::ROWS), g.n[2]);
+        hipLaunchKernelGGL((k_sweep<R, 1, MODE>), grid, dim3(256), 0, c->stream, a);
+    } else {
+        dim3 grid(blocks_for(g.n[0], SweepTile<2>::LANES), blocks_for(nif, SweepTile<2>::ROWS), g.n[1]);
+        hipLaunchKernelGGL((k_sweep<R, 2, MODE>), grid, dim3(256), 0, c->stream, a);
+    }
+}
+
+template <typename R>
+SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
+    SweepArgs<R> a{};
+    a.q = qsrc;
+    a.dF = (R*)c->dF[d];
+    // x: fsy, fsz; y: gsz, gsx; z: hsx, hsy
+    a.ct_a = (R*)c->ct[2 * d];
+    a.ct_b = (R*)c->ct[2 * d + 1];
+    if (d == 1) { a.ct_a = (R*)c->ct[3]; a.ct_b = (R*)c->ct[2]; }
+    a.g = c->g;
+    a.ph = c->ph;
+    a.mlp = c->mlp;
+    a.stats = (c->ind_mode == 2) ? c->istats : nullptr;
+    return a;
+}
+
 template <typename R>
 int launch_sweeps(howmhd_gpu_ctx* c, const double* qsrc) {
-    const Geom& g = c->g;
     for (int d = 0; d < 3; ++d) {
-        if (g.n[d] <= 1) continue;
-        SweepArgs<R> a;
-        a.q = qsrc;
-        a.dF = (R*)c->dF[d];
-        a.ct_a = (R*)c->ct[2 * d];
-        a.ct_b = (R*)c->ct[2 * d + 1];
-        // x: fsy, fsz; y: gsz, gsx; z: hsx, hsy
-        if (d == 1) { a.ct_a = (R*)c->ct[3]; a.ct_b = (R*)c->ct[2]; }
-        a.g = g;
-        a.ph = c->ph;
-        const int nif = g.per[d] ? g.n[d] : g.n[d] + 1;
+        if (c->g.n[d] <= 1) continue;
+        const SweepArgs<R> a = sweep_args<R>(c, d, qsrc);
         TimedLaunch tl(c, HOWMHD_K_SWEEP_X + d);
-        if (d == 0) {
-            dim3 grid(blocks_for(nif, SweepTile<0>::LANES), blocks_for(g.n[1], SweepTile<0>::ROWS), g.n[2]);
-            hipLaunchKernelGGL((k_sweep<R, 0>), grid, dim3(256), 0, c->stream, a);
-        } else if (d == 1) {
-            dim3 grid(blocks_for(g.n[0], SweepTile<1>::LANES), blocks_for(nif, SweepTile<1>::ROWS), g.n[2]);
-            hipLaunchKernelGGL((k_sweep<R, 1>), grid, dim3(256), 0, c->stream, a);
-        } else {
-            dim3 grid(blocks_for(g.n[0], SweepTile<2>::LANES), blocks_for(nif, SweepTile<2>::ROWS), g.n[1]);
-            hipLaunchKernelGGL((k_sweep<R, 2>), grid, dim3(256), 0, c->stream, a);
-        }
+        if (c->ind_mode) launch_sweep<R, SW_ROUTED>(c, d, a);
+        else launch_sweep<R, SW_PLAIN>(c, d, a);
         HCHECK(c, hipGetLastError());
     }
     return HOWMHD_OK;
+}
+
+// fp32 -> bf16 -> fp32, round to nearest even (the device's f2bf)
+float bf16_round(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+uint16_t bf16_bits(float f) {
+    const float r = bf16_round(f);
+    uint32_t u;
+    std::memcpy(&u, &r, 4);
+    return (uint16_t)(u >> 16);
 }
 
 template <typename R>
@@ -300,8 +336,8 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
         g_create_error = "howmhd_gpu_create: unknown precision";
         return HOWMHD_EINVAL;
     }
-    if (p->indicator != HOWMHD_INDICATOR_OFF) {
-        g_create_error = "howmhd_gpu_create: the learned indicator needs weights; use howmhd_gpu_set_indicator";
+    if (p->indicator != HOWMHD_INDICATOR_OFF && p->indicator != HOWMHD_INDICATOR_MLP) {
+        g_create_error = "howmhd_gpu_create: unknown indicator";
         return HOWMHD_EINVAL;
     }
     if (!(p->dx > 0 && p->dy > 0 && p->dz > 0) || !(p->gam > 1.0)) {
@@ -354,6 +390,8 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
         ok = ok && alloc(&c->O[t], c->ncell * c->rsize);
     }
     ok = ok && alloc((void**)&c->dmax, 4 * sizeof(unsigned long long));
+    ok = ok && alloc((void**)&c->mlp, sizeof(MlpDev));
+    ok = ok && alloc((void**)&c->istats, 64 * 4 * sizeof(unsigned long long));
     if (!ok) return cleanup_fail(HOWMHD_ENOMEM, "howmhd_gpu_create: device allocation failed");
     *out = c;
     return HOWMHD_OK;
@@ -374,6 +412,8 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
     for (int t = 0; t < 6; ++t)
         if (c->ct[t]) (void)hipFree(c->ct[t]);
     if (c->dmax) (void)hipFree(c->dmax);
+    if (c->mlp) (void)hipFree(c->mlp);
+    if (c->istats) (void)hipFree(c->istats);
     drain_timing(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -442,6 +482,8 @@ int howmhd_gpu_tstep(howmhd_gpu_ctx* c, double* dt) {
 int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
     if (!c) return HOWMHD_EINVAL;
     if (!c->faces_ready) return fail(c, HOWMHD_ESTATE, "howmhd_gpu_ssprk: call init_faces or upload_faces first");
+    if (c->p.indicator == HOWMHD_INDICATOR_MLP && !c->ind_mode)
+        return fail(c, HOWMHD_ESTATE, "howmhd_gpu_ssprk: indicator mlp needs weights (howmhd_gpu_set_indicator)");
     for (int iw = 1; iw <= 5; ++iw) {
         const int rc = (c->rsize == sizeof(float)) ? launch_stage<float>(c, iw, dt) : launch_stage<double>(c, iw, dt);
         if (rc) return rc;
@@ -564,6 +606,79 @@ int howmhd_gpu_synchronize(howmhd_gpu_ctx* c) {
 
 const char* howmhd_gpu_last_error(const howmhd_gpu_ctx* c) {
     return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int howmhd_gpu_collect_samples(howmhd_gpu_ctx* c, int dir, float* X, float* y, int64_t capacity, int64_t* count) {
+    if (!c || !X || !y || !count || dir < 0 || dir > 2) return HOWMHD_EINVAL;
+    *count = 0;
+    const int n = c->g.n[dir];
+    if (n <= 1) return HOWMHD_OK;
+    const int64_t need = (int64_t)(c->ncell / n) * (c->g.per[dir] ? n : n + 1);
+    if (capacity < need)
+        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_collect_samples: capacity " + std::to_string(capacity) +
+                                          " < " + std::to_string(need) + " interfaces");
+    if (c->rsize != sizeof(double))
+        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_collect_samples: labels come from the FP64 solver (precision fp64)");
+    // the sweep writes this direction's dF and CT terms too; the next stage
+    // recomputes them from its own source
+    SweepArgs<double> a = sweep_args<double>(c, dir, c->qbuf[c->qid[0]]);
+    a.feat = X;
+    a.label = y;
+    a.stats = nullptr;
+    launch_sweep<double, SW_COLLECT>(c, dir, a);
+    HCHECK(c, hipGetLastError());
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    *count = need;
+    return HOWMHD_OK;
+}
+
+int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float threshold, int enable) {
+    if (!c) return HOWMHD_EINVAL;
+    if (enable < 0 || enable > 2) return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: enable is 0, 1 or 2");
+    if (enable && !params) return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: null params");
+    if (enable && !(threshold > 0.f && threshold < 1.f))
+        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: threshold must lie in (0, 1)");
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    if (enable) {
+        for (int j = 0; j < HOWMHD_MLP_NPARAM; ++j)
+            if (!std::isfinite(params[j]))
+                return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: non-finite weight " + std::to_string(j));
+        MlpDev h;
+        std::memset(&h, 0, sizeof(h));
+        for (int k = 0; k < MLP_H; ++k)
+            for (int nn = 0; nn < MLP_H; ++nn) {
+                h.W1t[k][nn] = k < MLP_IN ? bf16_bits(params[MLP_OFF_W1 + nn * MLP_IN + k]) : 0;
+                h.W2t[k][nn] = bf16_bits(params[MLP_OFF_W2 + nn * MLP_H + k]);
+            }
+        for (int j = 0; j < MLP_H; ++j) {
+            h.b1[j] = params[MLP_OFF_B1 + j];
+            h.b2[j] = params[MLP_OFF_B2 + j];
+            h.w3[j] = bf16_round(params[MLP_OFF_W3 + j]);
+        }
+        h.b3 = params[MLP_OFF_B3];
+        h.logit_threshold = std::log(threshold / (1.f - threshold));
+        HCHECK(c, hipMemcpy(c->mlp, &h, sizeof(h), hipMemcpyHostToDevice));
+        HCHECK(c, hipMemset(c->istats, 0, 64 * 4 * sizeof(unsigned long long)));
+    }
+    c->ind_mode = enable;
+    return HOWMHD_OK;
+}
+
+int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* c, int64_t* flagged, int64_t* total, int64_t* waves_flagged,
+                               int64_t* waves_total) {
+    if (!c || !flagged || !total || !waves_flagged || !waves_total) return HOWMHD_EINVAL;
+    unsigned long long h[64 * 4];
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(h, c->istats, sizeof(h), hipMemcpyDeviceToHost));
+    HCHECK(c, hipMemset(c->istats, 0, sizeof(h)));
+    int64_t s[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 64; ++i)
+        for (int k = 0; k < 4; ++k) s[k] += (int64_t)h[4 * i + k];
+    *flagged = s[0];
+    *total = s[1];
+    *waves_flagged = s[2];
+    *waves_total = s[3];
+    return HOWMHD_OK;
 }
 
 const char* howmhd_gpu_build_kind(void) {

@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "mhd_math.hpp"
+#include "mlp.hpp"
 
 namespace howmhd {
 
@@ -89,6 +90,12 @@ template <> struct SweepTile<2> {
 #define HOWMHD_PROJ_BATCH 3
 #endif
 
+// Sweep variants: plain characteristic WENO5; the same plus indicator
+// samples (features and the WENO-weight label); routed by the learned
+// indicator (flagged interfaces: characteristic WENO5, others: linear
+// upwind-5 on the components).
+enum SweepMode { SW_PLAIN = 0, SW_COLLECT = 1, SW_ROUTED = 2 };
+
 template <typename R>
 struct SweepArgs {
     const double* q;    // source stage, 8 arrays of ncell
@@ -97,6 +104,13 @@ struct SweepArgs {
     R* ct_b;            // CT term "bsz": fsz (x), gsx (y), hsy (z)
     Geom g;
     Phys ph;
+    // SW_COLLECT: 18 features and the label per interface (sample_index)
+    float* feat;
+    float* label;
+    // SW_ROUTED: device weights; stats (null: not counted) holds 64 slots of
+    // (flagged, total, wavefronts flagged, wavefronts total), summed by the host
+    const MlpDev* mlp;
+    unsigned long long* stats;
 };
 
 // dF layout per direction: component-major, interface index t = 0..n along the
@@ -109,6 +123,17 @@ __device__ __forceinline__ long long dF_index(const Geom& g, int c, int t, int x
     if constexpr (DIR == 0) return (((long long)c * g.n[2] + z) * g.n[1] + y) * (g.n[0] + 1) + t;
     else if constexpr (DIR == 1) return (((long long)c * g.n[2] + z) * (g.n[1] + 1) + t) * g.n[0] + x;
     else return (((long long)c * (g.n[2] + 1) + t) * g.n[1] + y) * g.n[0] + x;
+}
+
+// Dense index of a sampled interface: the lines of the sweep, each with its
+// computed interfaces t = tbeg..n (tbeg = 1 periodic, 0 open) in order.
+template <int DIR>
+__device__ __forceinline__ long long sample_index(const Geom& g, int t, int x, int y, int z) {
+    const int tbeg = g.per[DIR] ? 1 : 0;
+    const long long nint = g.n[DIR] + 1 - tbeg, u = t - tbeg;
+    if constexpr (DIR == 0) return ((long long)z * g.n[1] + y) * nint + u;
+    else if constexpr (DIR == 1) return ((long long)z * nint + u) * g.n[0] + x;
+    else return (u * g.n[1] + y) * g.n[0] + x;
 }
 
 template <int M, typename R>
@@ -215,11 +240,12 @@ __device__ __forceinline__ void r_col_dyn(int m, R gam, const Eig<R>& e, R* Rc) 
     }
 }
 
-template <typename R, int DIR>
+template <typename R, int DIR, int MODE>
 __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> A) {
     using T = SweepTile<DIR>;
     __shared__ UF<R> luf[7][T::CELLS];
     __shared__ R lo[O_COUNT][T::CELLS];
+    __shared__ uint16_t mlp_act[MODE == SW_ROUTED ? 4 : 1][MODE == SW_ROUTED ? 64 * MLP_ROW : 1];
     const Geom& g = A.g;
     const int nd = g.n[DIR];
     const int per = g.per[DIR];
@@ -275,22 +301,60 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
 
     // ---- phase 2: one interface per thread
     int lane, row, t, x, y, z;
+    bool valid;
     if constexpr (DIR == 0) {
         lane = tid % T::LANES; row = tid / T::LANES;
         t = t0 + lane; x = 0; y = y0 + row; z = z0;
-        if (t > nd || y >= g.n[1]) return;
+        valid = (t <= nd && y < g.n[1]);
     } else {
         lane = tid % T::LANES; row = tid / T::LANES;
         t = t0 + row; x = x0 + lane;
         if constexpr (DIR == 1) { y = 0; z = z0; }
         else { y = y0; z = 0; }
-        if (t > nd || x >= g.n[0]) return;
+        valid = (t <= nd && x < g.n[0]);
     }
     auto slot = [&](int k) -> int {
         if constexpr (DIR == 0) return row * (T::LANES + 5) + lane + k;
         else return (row + k) * T::LANES + lane;
     };
+    // indicator features: the 6-cell stencils of rho, p and |B|
+    auto features = [&](float* f) {
+        float rr[6], pp[6], bb[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int s = valid ? slot(k) : 0;
+            const R bn = lo[O_BN][s], b1 = luf[4][s].u, b2 = luf[5][s].u;
+            rr[k] = (float)luf[0][s].u;
+            pp[k] = (float)lo[O_P][s];
+            bb[k] = sqrtf((float)(bn * bn + b1 * b1 + b2 * b2));
+        }
+        mlp_features(rr, pp, bb, f);
+    };
+    bool flagged = true;
+    if constexpr (MODE == SW_ROUTED) {
+        float f[MLP_IN];
+        features(f);
+        BFrag W1, W2;
+        load_bfrag(A.mlp->W1t, W1);
+        load_bfrag(A.mlp->W2t, W2);
+        const float zl = mlp_forward_wave(f, mlp_act[tid / 64], W1, W2, *A.mlp);
+        flagged = valid && (zl > A.mlp->logit_threshold);
+        if (A.stats) {
+            const unsigned long long fb = __ballot(flagged), vb = __ballot(valid);
+            if ((tid & 63) == 0 && vb) {
+                unsigned long long* st = A.stats + 4 * ((blockIdx.x + blockIdx.y + blockIdx.z) & 63);
+                atomicAdd(&st[0], (unsigned long long)__popcll(fb));
+                atomicAdd(&st[1], (unsigned long long)__popcll(vb));
+                atomicAdd(&st[2], fb ? 1ull : 0ull);
+                atomicAdd(&st[3], 1ull);
+            }
+        }
+    }
+    if (!valid) return;
 
+    R dF[7] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+    R maxdev = R(0);
+    if (MODE != SW_ROUTED || flagged) {
     // interface state from cells k = 2, 3: (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H)
     R u0[9], u1[9];
     {
@@ -322,7 +386,6 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
     // Waves are taken in pairs (0,1), (2,3), (4,5), (6): both waves of a pair
     // are projected in one pass over the stencil in LDS, and dF still gets
     // the waves added in order 0..6.
-    R dF[7] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
     const R amxv[7] = {amx.v0, amx.v1, amx.v2, amx.v3, amx.v4, amx.v5, amx.v6};
     sfor<0, 4>([&](auto pc) {
         constexpr int m0 = 2 * decltype(pc)::value;
@@ -356,9 +419,14 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
             }
             if ((k + 1) % HOWMHD_PROJ_BATCH == 0) asm volatile("" ::: "memory");
         }
-        const R Fsa = weno_wave<R>(Fa, qa, amxv[m0]);
-        R Fsb = R(0);
-        if constexpr (two) Fsb = weno_wave<R>(Fb, qb, amxv[m1]);
+        R Fsa, Fsb = R(0);
+        if constexpr (MODE == SW_COLLECT) {
+            Fsa = weno_wave_dev<R>(Fa, qa, amxv[m0], maxdev);
+            if constexpr (two) Fsb = weno_wave_dev<R>(Fb, qb, amxv[m1], maxdev);
+        } else {
+            Fsa = weno_wave<R>(Fa, qa, amxv[m0]);
+            if constexpr (two) Fsb = weno_wave<R>(Fb, qb, amxv[m1]);
+        }
         opaque_after(e, Fsa);
         if constexpr (two) opaque_after(e, Fsb);
         R Rc[7];
@@ -371,6 +439,27 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
             for (int r = 0; r < 7; ++r) dF[r] += Rc[r] * Fsb;
         }
     });
+    } else {
+        // linear upwind-5 with one LLF speed: the largest |v_n| + c_f over the
+        // stencil bounds every wave's speed
+        R alpha = R(0);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int s = slot(k);
+            alpha = rmax(alpha, rabs(lo[O_VN][s]) + lo[O_LF][s]);
+        }
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            R U[6], F[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const UF<R> v = luf[j][slot(k)];
+                U[k] = v.u;
+                F[k] = v.f;
+            }
+            dF[j] = upwind5<R>(F, U, alpha);
+        }
+    }
 
 #pragma unroll
     for (int c = 0; c < 7; ++c) A.dF[dF_index<DIR>(g, c, t, x, y, z)] = dF[c];
@@ -396,6 +485,14 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
         else ci = ((long long)(t - 1) * g.n[1] + y) * g.n[0] + x;
         A.ct_a[ci] = dF[4] + ftmp1;
         A.ct_b[ci] = dF[5] + ftmp2;
+    }
+    if constexpr (MODE == SW_COLLECT) {
+        float f[MLP_IN];
+        features(f);
+        const long long si = sample_index<DIR>(g, t, x, y, z);
+#pragma unroll
+        for (int j = 0; j < MLP_IN; ++j) A.feat[si * MLP_IN + j] = f[j];
+        A.label[si] = (maxdev > R(0.05)) ? 1.f : 0.f;
     }
 }
 

@@ -466,4 +466,46 @@ __device__ __forceinline__ R weno_wave(const R* Fsk, const R* qsk, R amx) {
     return first - second + third;
 }
 
+// weno_wave that also raises `dev` to the largest |w_r - d_r| of both sides,
+// d = (0.1, 0.6, 0.3) (the label of the learned indicator; weights of
+// src/ssprk.cpp:903-908, 926-931, w1 = 1 - w0 - w2).
+template <typename R>
+__device__ __forceinline__ R weno_wave_dev(const R* Fsk, const R* qsk, R amx, R& dev) {
+    R dF[5], dq[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        dF[k] = Fsk[k + 1] - Fsk[k];
+        dq[k] = qsk[k + 1] - qsk[k];
+    }
+    const R first = div_c<R>(-Fsk[1] + R(7) * Fsk[2] + R(7) * Fsk[3] - Fsk[4], R(12), R(1.0 / 12.0));
+    R w0, w2;
+    const R second = weno_side<R>(R(0.5) * (dF[0] + amx * dq[0]), R(0.5) * (dF[1] + amx * dq[1]),
+                                  R(0.5) * (dF[2] + amx * dq[2]), R(0.5) * (dF[3] + amx * dq[3]), &w0, &w2);
+    dev = rmax(dev, rmax(rabs(w0 - R(0.1)), rmax(rabs(w2 - R(0.3)), rabs(R(1) - w0 - w2 - R(0.6)))));
+    const R third = weno_side<R>(R(0.5) * (dF[4] - amx * dq[4]), R(0.5) * (dF[3] - amx * dq[3]),
+                                 R(0.5) * (dF[2] - amx * dq[2]), R(0.5) * (dF[1] - amx * dq[1]), &w0, &w2);
+    dev = rmax(dev, rmax(rabs(w0 - R(0.1)), rmax(rabs(w2 - R(0.3)), rabs(R(1) - w0 - w2 - R(0.6)))));
+    return first - second + third;
+}
+
+// Linear upwind-5 flux of one component with one LLF speed alpha: the WENO5
+// flux of weno_wave with the ideal weights (w0 = 0.1, w2 = 0.3), where both
+// corrections reduce to (a - 3b + 3c - d)/30.  With one alpha for all waves the
+// characteristic projection cancels (R L = I), so it runs on the physical
+// components directly.
+template <typename R>
+__device__ __forceinline__ R upwind5(const R* F, const R* U, R alpha) {
+    R dF[5], dU[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        dF[k] = F[k + 1] - F[k];
+        dU[k] = U[k + 1] - U[k];
+    }
+    const R first = (-F[1] + R(7) * F[2] + R(7) * F[3] - F[4]) * R(1.0 / 12.0);
+    const R a = dF[0] + alpha * dU[0], b = dF[1] + alpha * dU[1], c = dF[2] + alpha * dU[2], d = dF[3] + alpha * dU[3];
+    const R e = dF[4] - alpha * dU[4], f = dF[3] - alpha * dU[3], g = dF[2] - alpha * dU[2], h = dF[1] - alpha * dU[1];
+    // 0.5 (a - 3b + 3c - d)/30 for each side
+    return first - (a - R(3) * b + R(3) * c - d) * R(1.0 / 60.0) + (e - R(3) * f + R(3) * g - h) * R(1.0 / 60.0);
+}
+
 }  // namespace howmhd

@@ -1,0 +1,136 @@
+// Learned smoothness indicator: an 18 -> 32 -> 32 -> 1 MLP (ReLU, ReLU,
+// sigmoid) evaluated with bf16 MFMA (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation) by whole wavefronts: the 64 lanes of a wavefront are 64
+// interfaces (or training samples), split into four 16-row tiles.
+//
+// Inputs: the 6-cell stencils (the WENO5 stencil of the interface) of rho, p
+// and |B|, each mapped to [0,1] by its own min and max over the 6 cells.
+// Activations between layers are rounded to bf16 (they are MFMA operands);
+// the last layer is an FP32 dot product with the bf16 output weights.
+//
+// MFMA 16x16x16 operand layout (lane l, element v = 0..3):
+//   A (16 x 16, rows x k):  A[l % 16][4*(l/16) + v]
+//   B (16 x 16, k x cols):  B[4*(l/16) + v][l % 16]
+//   D (16 x 16, rows x cols): D[4*(l/16) + v][l % 16]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace howmhd {
+
+constexpr int MLP_IN = 18, MLP_H = 32;
+// master (FP32) parameter vector: W1[32][18], b1[32], W2[32][32], b2[32], w3[32], b3
+constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = 576, MLP_OFF_W2 = 608, MLP_OFF_B2 = 1632, MLP_OFF_W3 = 1664,
+              MLP_OFF_B3 = 1696, MLP_NPARAM = 1697;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// fp32 -> bf16, round to nearest even (inputs here are finite)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+// Device form of the weights for inference: bf16 B operands of layers 1 and
+// 2 (transposed, layer-1 k padded 18 -> 32 with zeros), FP32 biases, bf16
+// output weights held as floats.
+struct MlpDev {
+    uint16_t W1t[MLP_H][MLP_H];  // W1t[k][n] = W1[n][k], k < 18
+    uint16_t W2t[MLP_H][MLP_H];  // W2t[k][n] = W2[n][k]
+    float b1[MLP_H], b2[MLP_H], w3[MLP_H];
+    float b3;
+    float logit_threshold;       // flag when z3 > threshold (0 <-> p > 0.5)
+};
+
+// B-operand fragments of one 32x32 bf16 matrix Wt[k][n]: [k-step s][n-tile u]
+struct BFrag {
+    bf16x4 f[2][2];
+};
+
+__device__ __forceinline__ void load_bfrag(const uint16_t (*Wt)[MLP_H], BFrag& b) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b.f[s][u][v] = (short)Wt[16 * s + 4 * (l / 16) + v][16 * u + (l % 16)];
+}
+
+// Scratch rows of one wavefront: 64 rows x 32 bf16 (+2 pad to stagger banks).
+constexpr int MLP_ROW = MLP_H + 2;
+
+// One dense layer on the 64 rows of a wavefront: dst <- relu(src * Wt + b),
+// bf16 rows of MLP_ROW, four 16-row tiles.  src may equal dst: every lane
+// reads its fragments of a tile before any lane writes that tile.
+__device__ __forceinline__ void layer_relu(const uint16_t* src, uint16_t* dst, const BFrag& W, const float* bias) {
+    const int l = threadIdx.x & 63;
+    const int r16 = l % 16, kq = 4 * (l / 16);
+    const float bl0 = bias[r16], bl1 = bias[16 + r16];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        bf16x4 a[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[s][v] = (short)src[(16 * t + r16) * MLP_ROW + 16 * s + kq + v];
+        f32x4 d[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const float b = u ? bl1 : bl0;
+            f32x4 c = {b, b, b, b};
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], W.f[0][u], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], W.f[1][u], c, 0, 0, 0);
+            d[u] = c;
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float z = d[u][v];
+                dst[(16 * t + kq + v) * MLP_ROW + 16 * u + r16] = f2bf(z > 0.f ? z : 0.f);
+            }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Features of one interface from its 6-cell stencils of rho, p and |B|.
+__device__ __forceinline__ void mlp_features(const float* rho, const float* p, const float* b, float* f) {
+    const float* src[3] = {rho, p, b};
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        float lo = src[g][0], hi = src[g][0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) {
+            lo = fminf(lo, src[g][k]);
+            hi = fmaxf(hi, src[g][k]);
+        }
+        const float span = hi - lo;
+        const float inv = span > 1e-12f * fmaxf(fabsf(hi), fabsf(lo)) + 1e-30f ? 1.0f / span : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) f[6 * g + k] = (src[g][k] - lo) * inv;
+    }
+}
+
+// Forward pass of the wavefront: this lane's 18 features in, its output
+// logit z3 back.  `act` is this wavefront's 64 x MLP_ROW bf16 scratch.
+__device__ __forceinline__ float mlp_forward_wave(const float* feat, uint16_t* act, const BFrag& W1, const BFrag& W2,
+                                                  const MlpDev& m) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < MLP_H; ++j) act[l * MLP_ROW + j] = (j < MLP_IN) ? f2bf(feat[j < MLP_IN ? j : 0]) : 0;
+    __builtin_amdgcn_wave_barrier();
+    layer_relu(act, act, W1, m.b1);
+    layer_relu(act, act, W2, m.b2);
+    float z = m.b3;
+#pragma unroll
+    for (int j = 0; j < MLP_H; ++j) z += bf2f(act[l * MLP_ROW + j]) * m.w3[j];
+    return z;
+}
+
+}  // namespace howmhd

@@ -9,6 +9,7 @@
 // zone-cycles/s of the step loop.
 #include "howmhd_gpu.h"
 #include "howmhd_host.h"
+#include "howmhd_mlp.h"
 
 #include <chrono>
 #include <cmath>
@@ -103,6 +104,15 @@ int main(int argc, char* argv[]) {
 
     howmhd_gpu_ctx* ctx = nullptr;
     if (howmhd_gpu_create(&gp, &ctx) != 0) die(std::string("Error: ") + howmhd_gpu_last_error(nullptr));
+    if (gp.indicator == HOWMHD_INDICATOR_MLP) {
+        if (par.indicator_weights[0] == '\0') die("Error: indicator mlp needs indicator_weights <file>");
+        std::vector<float> w(HOWMHD_MLP_NPARAM);
+        float thr = 0.5f;
+        if (howmhd_host_load_indicator(par.indicator_weights, w.data(), HOWMHD_MLP_NPARAM, &thr, err, sizeof(err)) != 0)
+            die(err);
+        check(howmhd_gpu_set_indicator(ctx, w.data(), thr, 1), ctx, "set_indicator");
+        std::cout << "indicator weights  = " << par.indicator_weights << " (threshold " << thr << ")\n";
+    }
     check(howmhd_gpu_upload_q0(ctx, q0.data()), ctx, "upload_q0");
     check(howmhd_gpu_init_faces(ctx), ctx, "init_faces");
 

@@ -9,8 +9,10 @@
 // after nstep hydro steps" (the reference parses it and never reads it).
 #include "howmhd_host.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -149,6 +151,60 @@ extern "C" int howmhd_host_gpu_params(const howmhd_run* r, howmhd_params* p, cha
     else if (std::strcmp(r->indicator, "mlp") == 0) p->indicator = HOWMHD_INDICATOR_MLP;
     else {
         fail(errbuf, errlen, std::string("Error: unknown indicator = ") + r->indicator + " (off, mlp)");
+        return HOWMHD_EINVAL;
+    }
+    return HOWMHD_OK;
+}
+
+extern "C" int howmhd_host_load_indicator(const char* path, float* params, int nparam, float* threshold,
+                                          char* errbuf, int errlen) {
+    if (!path || !params || !threshold) {
+        fail(errbuf, errlen, "howmhd_host_load_indicator: null argument");
+        return HOWMHD_EINVAL;
+    }
+    std::ifstream f(path);
+    if (!f) {
+        fail(errbuf, errlen, std::string("Error: cannot open indicator_weights file: ") + path);
+        return HOWMHD_EINVAL;
+    }
+    std::string line;
+    bool header = false;
+    int count = 0;
+    while (std::getline(f, line)) {
+        const size_t b = line.find_first_not_of(" \t\r");
+        if (b == std::string::npos || line[b] == '#') continue;
+        std::istringstream in(line.substr(b));
+        if (!header) {
+            std::string tag;
+            int n = 0;
+            double thr = 0.0;
+            if (!(in >> tag >> n >> thr) || tag != "howmhd-mlp") {
+                fail(errbuf, errlen, std::string("Error: ") + path + ": header 'howmhd-mlp <nparam> <threshold>' expected");
+                return HOWMHD_EINVAL;
+            }
+            if (n != nparam) {
+                fail(errbuf, errlen, std::string("Error: ") + path + ": " + std::to_string(n) + " parameters, expected " +
+                                         std::to_string(nparam));
+                return HOWMHD_EINVAL;
+            }
+            if (!(thr > 0.0 && thr < 1.0)) {
+                fail(errbuf, errlen, std::string("Error: ") + path + ": threshold must lie in (0, 1)");
+                return HOWMHD_EINVAL;
+            }
+            *threshold = (float)thr;
+            header = true;
+            continue;
+        }
+        double v = 0.0;
+        if (!(in >> v) || !std::isfinite(v) || count >= nparam) {
+            fail(errbuf, errlen, std::string("Error: ") + path + ": bad or extra value: " + line);
+            return HOWMHD_EINVAL;
+        }
+        params[count++] = (float)v;
+    }
+    if (!header || count != nparam) {
+        fail(errbuf, errlen, std::string("Error: ") + path + ": " + std::to_string(count) + " of " +
+                                 std::to_string(nparam) + " parameters read");
         return HOWMHD_EINVAL;
     }
     return HOWMHD_OK;

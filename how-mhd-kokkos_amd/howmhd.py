@@ -111,7 +111,20 @@ GPU_FUNCS = {
     "howmhd_gpu_get_timing": (C.c_int, [C.c_void_p, C.POINTER(Timing)]),
     "howmhd_gpu_last_error": (C.c_char_p, [C.c_void_p]),
     "howmhd_gpu_build_kind": (C.c_char_p, []),
+    # learned indicator, include/howmhd_mlp.h
+    "howmhd_mlp_infer": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "howmhd_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_float, C.c_float,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]),
+    "howmhd_mlp_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
+                                  C.c_float, C.c_float, C.c_float, C.c_void_p]),
+    "howmhd_gpu_collect_samples": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64,
+                                             C.POINTER(C.c_int64)]),
+    "howmhd_gpu_set_indicator": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_int]),
+    "howmhd_gpu_indicator_stats": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int64)] * 4),
 }
+
+MLP_NPARAM = 1697
+MLP_NFEAT = 18
 
 HOST_FUNCS = {
     "howmhd_host_parse": (C.c_int, [C.c_char_p, C.POINTER(Run), C.c_char_p, C.c_int]),
@@ -119,6 +132,8 @@ HOST_FUNCS = {
     "howmhd_host_init_problem": (C.c_int, [C.POINTER(Run), _DP, C.c_char_p, C.c_int]),
     "howmhd_host_problem_names": (C.c_char_p, []),
     "howmhd_host_output": (C.c_int, [C.POINTER(Run), _DP, C.c_int, C.c_double, C.c_char_p, C.c_char_p, C.c_int]),
+    "howmhd_host_load_indicator": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
+                                             C.c_char_p, C.c_int]),
 }
 
 
@@ -182,6 +197,18 @@ def write_output(run: Run, q0: np.ndarray, step: int, time: float, directory: st
     if lib.howmhd_host_output(C.byref(run), _ptr(np.ascontiguousarray(q0)), step, time, directory.encode(), err,
                               len(err)) != 0:
         raise HowmhdError(err.value.decode())
+
+
+def load_indicator(path) -> tuple[np.ndarray, float]:
+    """(params, threshold) of an indicator weights file (howmhd_host_load_indicator)."""
+    lib = host_lib()
+    P = np.zeros(MLP_NPARAM, dtype=np.float32)
+    thr = C.c_float()
+    err = C.create_string_buffer(4096)
+    if lib.howmhd_host_load_indicator(str(path).encode(), P.ctypes.data_as(C.POINTER(C.c_float)), MLP_NPARAM,
+                                      C.byref(thr), err, len(err)) != 0:
+        raise HowmhdError(err.value.decode())
+    return P, thr.value
 
 
 def make_input(**kw) -> str:
@@ -255,6 +282,43 @@ class GpuSolver:
         self._check(self.lib.howmhd_gpu_get_timing(self.h, C.byref(t)), "get_timing")
         return {k: (t.ms[i], t.launches[i]) for i, k in enumerate(KERNELS)}
 
+    # ---- learned indicator (include/howmhd_mlp.h)
+    def n_samples(self, d: int) -> int:
+        """Interfaces the sweep along axis d computes (0 for a degenerate axis)."""
+        n = (self.p.nx, self.p.ny, self.p.nz)[d]
+        if n <= 1:
+            return 0
+        cells = self.p.nx * self.p.ny * self.p.nz
+        return cells // n * (n if self.p.bc[d] == BC["periodic"] else n + 1)
+
+    def collect_samples(self, d: int, X, y) -> int:
+        """Features (n, 18) and labels (n,) of the current state into the
+        float32 device tensors X, y (torch); returns the sample count."""
+        _check_dev(X, (None, MLP_NFEAT))
+        _check_dev(y, (X.shape[0],))
+        _torch().cuda.synchronize()
+        cnt = C.c_int64()
+        self._check(self.lib.howmhd_gpu_collect_samples(self.h, d, X.data_ptr(), y.data_ptr(), X.shape[0],
+                                                        C.byref(cnt)), "collect_samples")
+        return cnt.value
+
+    def set_indicator(self, params, threshold: float = 0.5, enable: int = 1):
+        """Routing on (1), on and counted (2) or off (0) with host weights."""
+        if params is None:
+            self._check(self.lib.howmhd_gpu_set_indicator(self.h, None, threshold, enable), "set_indicator")
+            return
+        P = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+        assert P.shape == (MLP_NPARAM,)
+        self._check(self.lib.howmhd_gpu_set_indicator(self.h, P.ctypes.data_as(C.POINTER(C.c_float)), threshold,
+                                                      enable), "set_indicator")
+
+    def indicator_stats(self) -> dict:
+        v = [C.c_int64() for _ in range(4)]
+        self._check(self.lib.howmhd_gpu_indicator_stats(self.h, *[C.byref(x) for x in v]), "indicator_stats")
+        f, t, wf, wt = (x.value for x in v)
+        return {"flagged": f, "total": t, "waves_flagged": wf, "waves_total": wt,
+                "flagged_fraction": f / t if t else 0.0, "wave_fraction": wf / wt if wt else 0.0}
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.howmhd_gpu_destroy(self.h)
@@ -271,6 +335,64 @@ class GpuSolver:
 
     def __exit__(self, *a):
         self.close()
+
+
+# ------------------------------------------------------- indicator kernels
+def _torch():
+    import torch
+    return torch
+
+
+def _check_dev(t, shape):
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise HowmhdError("expected a contiguous float32 CUDA tensor")
+    if len(shape) != t.dim() or any(s is not None and s != d for s, d in zip(shape, t.shape)):
+        raise HowmhdError(f"tensor shape {tuple(t.shape)} does not match {shape}")
+
+
+def _stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def mlp_infer(X, P, exact: bool = False):
+    """Logits z3 of the samples X (n, 18) with parameters P (1697,), both
+    float32 CUDA tensors; runs on torch's current stream."""
+    torch = _torch()
+    _check_dev(X, (None, MLP_NFEAT))
+    _check_dev(P, (MLP_NPARAM,))
+    out = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+    rc = gpu_lib(exact).howmhd_mlp_infer(X.data_ptr(), X.shape[0], P.data_ptr(), out.data_ptr(), _stream())
+    if rc:
+        raise HowmhdError(f"howmhd_mlp_infer failed ({rc})")
+    return out
+
+
+def mlp_grad(X, y, P, pos_weight=1.0, scale=1.0, G=None, loss=None, exact: bool = False):
+    """Adds the gradient (and loss) of the weighted BCE over (X, y) to G
+    (and loss); zero-initialised tensors are made when not given."""
+    torch = _torch()
+    _check_dev(X, (None, MLP_NFEAT))
+    _check_dev(y, (X.shape[0],))
+    _check_dev(P, (MLP_NPARAM,))
+    if G is None:
+        G = torch.zeros(MLP_NPARAM, dtype=torch.float32, device=X.device)
+    if loss is None:
+        loss = torch.zeros(1, dtype=torch.float32, device=X.device)
+    rc = gpu_lib(exact).howmhd_mlp_grad(X.data_ptr(), y.data_ptr(), X.shape[0], P.data_ptr(), pos_weight, scale,
+                                        G.data_ptr(), loss.data_ptr(), _stream())
+    if rc:
+        raise HowmhdError(f"howmhd_mlp_grad failed ({rc})")
+    return G, loss
+
+
+def mlp_adam(P, M, V, G, t: int, lr: float, beta1=0.9, beta2=0.999, eps=1e-8, exact: bool = False):
+    for a in (P, M, V, G):
+        _check_dev(a, (MLP_NPARAM,))
+    rc = gpu_lib(exact).howmhd_mlp_adam(P.data_ptr(), M.data_ptr(), V.data_ptr(), G.data_ptr(), t, lr, beta1, beta2,
+                                        eps, _stream())
+    if rc:
+        raise HowmhdError(f"howmhd_mlp_adam failed ({rc})")
 
 
 def build_kind(exact: bool = False) -> str:

@@ -58,6 +58,13 @@ const char* howmhd_host_problem_names(void);
 int howmhd_host_output(const howmhd_run* run, const double* q0, int step, double time,
                        const char* dir, char* errbuf, int errlen);
 
+/* Reads a learned-indicator weights file (the `indicator_weights` input key;
+ * no counterpart in the reference): text, '#' lines are comments, then the
+ * header "howmhd-mlp <nparam> <threshold>" and nparam FP32 values.  nparam
+ * must equal HOWMHD_MLP_NPARAM (1697) and the threshold lie in (0, 1). */
+int howmhd_host_load_indicator(const char* path, float* params, int nparam, float* threshold,
+                               char* errbuf, int errlen);
+
 #ifdef __cplusplus
 }
 #endif

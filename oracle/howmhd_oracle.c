@@ -34,6 +34,7 @@ struct howmhd_ref_ctx {
     double* bz;
     double* ct[12];        /* fsy fsz gsx gsz hsx hsy Ox2 Oy2 Oz2 Ox Oy Oz */
     double dt, dtdx, dtdy, dtdz;
+    int route;             /* 0: characteristic WENO5 everywhere; 1: linear upwind-5 everywhere */
 };
 
 enum { FSY, FSZ, GSX, GSZ, HSX, HSY, OX2, OY2, OZ2, OX, OY, OZ };
@@ -210,6 +211,7 @@ static void bound_faces(howmhd_ref_ctx* c, int s) {
 /* ====================================================== one-line pipeline == */
 typedef struct {
     int nn, N;
+    int route;   /* 1: every interface takes the linear upwind-5 flux */
     double *qx, *ux, *ax, *Fx, *Lx, *Rx, *dF, *bsy, *bsz;
 } line_ws;
 
@@ -223,6 +225,7 @@ typedef struct {
 
 static int ws_alloc(line_ws* w, int nn) {
     w->nn = nn;
+    w->route = 0;
     w->N = nn + 6;
     const size_t N = (size_t)w->N;
     w->qx = (double*)calloc(8 * N, sizeof(double));
@@ -544,12 +547,49 @@ static double weno_wave(const double* Fsk, const double* qsk, double amx, double
     return first - second + third;
 }
 
+/* Linear upwind-5 flux of one component with one LLF speed alpha (the
+ * interface flux of the learned-indicator form for unflagged interfaces; no
+ * counterpart in the reference).  It is weno_wave with the ideal weights
+ * (0.1, 0.6, 0.3), where both corrections reduce to (a - 3b + 3c - d)/30 on
+ * half-differences; with one alpha for all waves the characteristic
+ * projection cancels, so it acts on the components directly.  Evaluation
+ * order as the HIP kernel. */
+static double upwind5(const double* F, const double* U, double alpha) {
+    double dF[5], dU[5];
+    for (int k = 0; k < 5; ++k) {
+        dF[k] = F[k + 1] - F[k];
+        dU[k] = U[k + 1] - U[k];
+    }
+    const double first = (-F[1] + 7.0 * F[2] + 7.0 * F[3] - F[4]) * (1.0 / 12.0);
+    const double a = dF[0] + alpha * dU[0], b = dF[1] + alpha * dU[1];
+    const double c = dF[2] + alpha * dU[2], d = dF[3] + alpha * dU[3];
+    const double e = dF[4] - alpha * dU[4], f = dF[3] - alpha * dU[3];
+    const double g = dF[2] - alpha * dU[2], h = dF[1] - alpha * dU[1];
+    return first - (a - 3.0 * b + 3.0 * c - d) * (1.0 / 60.0) + (e - 3.0 * f + 3.0 * g - h) * (1.0 / 60.0);
+}
+
+/* slot of conserved component j (rho, Mn, Mt1, Mt2, Bt1, Bt2, E) in qx */
+static const int QSLOT[7] = {0, 1, 2, 3, 5, 6, 7};
+
 /* weno_x_all, src/ssprk.cpp:825-997: flux (serial wave order), update and CT terms */
 static void line_weno(line_ws* w, double dtdl) {
     const int nn = w->nn;
     double Fsk[6], qsk[6];
     memset(w->dF, 0, sizeof(double) * 7 * (size_t)(nn + 1));
     for (int i = 0; i <= nn; ++i) {
+        if (w->route == 1) {
+            /* alpha = max over the 6 cells of |vn| + c_f = max over waves of |lambda| */
+            double alpha = 0.0;
+            for (int s = -2; s <= 3; ++s) alpha = fmax(alpha, fmax(fabs(WA(w, 0, i + s + 2)), fabs(WA(w, 6, i + s + 2))));
+            for (int j = 0; j < 7; ++j) {
+                for (int ks = 0; ks < 6; ++ks) {
+                    Fsk[ks] = WF(w, j, i + ks);
+                    qsk[ks] = WQ(w, QSLOT[j], i + ks);
+                }
+                WD(w, j, i) = upwind5(Fsk, qsk, alpha);
+            }
+            continue;
+        }
         for (int m = 0; m < 7; ++m) {
             double amx = 0.0;
             for (int s = -2; s <= 3; ++s) amx = fmax(amx, fabs(WA(w, m, i + s + 2)));
@@ -637,6 +677,7 @@ static void sweep(howmhd_ref_ctx* c, int dir, int iw, rkc k, double dtdl) {
                         WQ(&w, m, s) = Q(c, iw - 1, PERM[dir][m], idx[0], idx[1], idx[2]);
                     }
                 /* L and R are read only at interfaces 2..nn+2, all rewritten per line */
+                w.route = c->route;
                 line_pipeline(&w, gam, rhomin, pgmin, dtdl);
                 for (int i = 0; i <= nn; ++i) {
                     idx[dir] = i + 2;
@@ -1047,4 +1088,51 @@ double howmhd_ref_weno_wave(const double* Fsk, const double* qsk, double amx) {
 double howmhd_ref_weno_wave_weights(const double* Fsk, const double* qsk, double amx,
                                     double* wplus, double* wminus) {
     return weno_wave(Fsk, qsk, amx, wplus, wminus);
+}
+
+double howmhd_ref_upwind5(const double* F, const double* U, double alpha) { return upwind5(F, U, alpha); }
+
+int howmhd_ref_set_route(howmhd_ref_ctx* c, int route) {
+    if (!c || route < 0 || route > 1) return HOWMHD_EINVAL;
+    c->route = route;
+    return 0;
+}
+
+/* Label of the learned indicator for every interface of one line: the
+ * largest |w_r - d_r| over waves, sides and r (d = 0.1, 0.6, 0.3; w1 taken as
+ * 1 - w0 - w2, src/ssprk.cpp:903-908, 926-931), from the projections of
+ * line_weno. */
+int howmhd_ref_line_maxdev(double gam, double rhomin, double pgmin, int nn, const double* qline, double* maxdev) {
+    line_ws w;
+    if (nn < 1 || ws_alloc(&w, nn) != 0) return HOWMHD_EINVAL;
+    memcpy(w.qx, qline, sizeof(double) * 8 * (size_t)w.N);
+    line_primit(&w, gam, rhomin, pgmin);
+    line_center(&w, gam);
+    line_boundary(&w, gam);
+    double Fsk[6], qsk[6], wp[3], wm[3];
+    for (int i = 0; i <= nn; ++i) {
+        double dev = 0.0;
+        for (int m = 0; m < 7; ++m) {
+            double amx = 0.0;
+            for (int s = -2; s <= 3; ++s) amx = fmax(amx, fabs(WA(&w, m, i + s + 2)));
+            for (int ks = 0; ks < 6; ++ks) {
+                const int c = i + ks;
+                Fsk[ks] = 0.0;
+                qsk[ks] = 0.0;
+                for (int col = 0; col < 7; ++col) {
+                    Fsk[ks] += WL(&w, m, col, i + 2) * WF(&w, col, c);
+                    qsk[ks] += WL(&w, m, col, i + 2) * WQ(&w, QSLOT[col], c);
+                }
+            }
+            (void)weno_wave(Fsk, qsk, amx, wp, wm);
+            const double* ws[2] = {wp, wm};
+            for (int sd = 0; sd < 2; ++sd) {
+                const double w0 = ws[sd][0], w2 = ws[sd][2], w1 = 1.0 - w0 - w2;
+                dev = fmax(dev, fmax(fabs(w0 - 0.1), fmax(fabs(w2 - 0.3), fabs(w1 - 0.6))));
+            }
+        }
+        maxdev[i] = dev;
+    }
+    ws_free(&w);
+    return 0;
 }

@@ -64,6 +64,20 @@ double howmhd_ref_weno_wave(const double* Fsk, const double* qsk, double amx);
 double howmhd_ref_weno_wave_weights(const double* Fsk, const double* qsk,
                                     double amx, double* wplus, double* wminus);
 
+/* --- learned-indicator form (no counterpart in the reference) --- */
+
+/* Linear upwind-5 LLF flux of one component from 6 stencil values. */
+double howmhd_ref_upwind5(const double* F, const double* U, double alpha);
+
+/* 0: characteristic WENO5 at every interface (default); 1: linear upwind-5
+ * at every interface (the indicator form with no interface flagged). */
+int howmhd_ref_set_route(howmhd_ref_ctx* c, int route);
+
+/* Indicator label source for one line (qline as in howmhd_ref_line_sweep,
+ * not modified): maxdev[i], i = 0..nn, the largest deviation of the WENO
+ * nonlinear weights from the ideal weights at interface i. */
+int howmhd_ref_line_maxdev(double gam, double rhomin, double pgmin, int nn, const double* qline, double* maxdev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,9 @@ def lib() -> C.CDLL:
             "howmhd_ref_eigensystem": (None, [C.c_double, _DP, _DP, _DP, _DP]),
             "howmhd_ref_weno_wave": (C.c_double, [_DP, _DP, C.c_double]),
             "howmhd_ref_weno_wave_weights": (C.c_double, [_DP, _DP, C.c_double, _DP, _DP]),
+            "howmhd_ref_upwind5": (C.c_double, [_DP, _DP, C.c_double]),
+            "howmhd_ref_set_route": (C.c_int, [C.c_void_p, C.c_int]),
+            "howmhd_ref_line_maxdev": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, _DP, _DP]),
         }
         for fn, (res, args) in table.items():
             f = getattr(_lib, fn)
@@ -105,6 +108,10 @@ class RefSolver:
         self.lib.howmhd_ref_download_faces(self.h, _ptr(out))
         return out
 
+    def set_route(self, route: int):
+        """0: characteristic WENO5 everywhere; 1: linear upwind-5 everywhere."""
+        assert self.lib.howmhd_ref_set_route(self.h, route) == 0
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.howmhd_ref_destroy(self.h)
@@ -147,3 +154,18 @@ def weno_wave(Fsk, qsk, amx, weights=False):
     wm = np.zeros(3)
     fs = lib().howmhd_ref_weno_wave_weights(_ptr(F), _ptr(q), amx, _ptr(wp), _ptr(wm))
     return fs, wp, wm
+
+
+def upwind5(F, U, alpha):
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return lib().howmhd_ref_upwind5(_ptr(F), _ptr(U), alpha)
+
+
+def line_maxdev(qline: np.ndarray, gam: float, rhomin=1e-12, pgmin=1e-12):
+    """Largest WENO-weight deviation from the ideal weights at the nn+1 interfaces of a line."""
+    q = np.ascontiguousarray(qline, dtype=np.float64).copy()
+    nn = q.shape[1] - 6
+    out = np.zeros(nn + 1)
+    assert lib().howmhd_ref_line_maxdev(gam, rhomin, pgmin, nn, _ptr(q), _ptr(out)) == 0
+    return out

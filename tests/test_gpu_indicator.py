@@ -1,0 +1,232 @@
+"""The learned smoothness indicator on the GPU (needs a GPU).
+
+  * howmhd_mlp_infer / howmhd_mlp_grad / howmhd_mlp_adam against the NumPy
+    restatement of the same numerics (this also pins the MFMA operand layout:
+    a transposed fragment gives O(1) differences);
+  * sample collection: features and labels of every interface against the
+    restatement's primitives and WENO weights (periodic and open axes);
+  * routing: every interface flagged == the plain method bit for bit, none
+    flagged == the restatement's linear route bit for bit (FP-contraction-off
+    build); a mixed routing keeps mass, momentum, energy and the CT face
+    divergence to round-off, and FP32 sweeps run routed.
+"""
+import numpy as np
+import pytest
+
+import howmhd as H
+import mlp_oracle as M
+import oracle_ctypes as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _params(seed, b3=None):
+    rng = np.random.default_rng(seed)
+    P = M.init_params(seed)
+    P[M.OFF_B1:M.OFF_W2] = rng.normal(0, 0.2, 32)
+    P[M.OFF_B2:M.OFF_W3] = rng.normal(0, 0.2, 32)
+    if b3 is not None:
+        P[M.OFF_W3:M.OFF_B3] = 0.0
+        P[M.OFF_B3] = b3
+    return P
+
+
+@pytest.mark.parametrize("n", [64, 1000, 10007])
+def test_mlp_infer_matches_restatement(n, require_gpu):
+    torch = _torch()
+    print(f"[mlp infer] n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    X = rng.uniform(0, 1, (n, 18)).astype(np.float32)
+    P = _params(1)
+    z = H.mlp_infer(torch.from_numpy(X).cuda(), torch.from_numpy(P).cuda()).cpu().numpy()
+    zr, _ = M.forward(P, X)
+    d = np.abs(z - zr)
+    scale = 1 + np.abs(zr)
+    print("infer max |dz|:", d.max(), " mean:", d.mean(), " mean |z|:", np.abs(zr).mean())
+    assert np.all(d <= 2e-2 * scale)
+    assert np.mean(d <= 1e-4 * scale) >= 0.99
+    assert d.mean() <= 1e-3
+
+
+def test_mlp_grad_matches_restatement(require_gpu):
+    torch = _torch()
+    n = 5000
+    rng = np.random.default_rng(7)
+    X = rng.uniform(0, 1, (n, 18)).astype(np.float32)
+    y = (rng.uniform(size=n) < 0.3).astype(np.float32)
+    P = _params(2)
+    G, loss = H.mlp_grad(torch.from_numpy(X).cuda(), torch.from_numpy(y).cuda(), torch.from_numpy(P).cuda(),
+                         pos_weight=2.5, scale=1.0 / n)
+    G, loss = G.cpu().numpy().astype(np.float64), float(loss.cpu().numpy()[0])
+    lr, Gr = M.loss_and_grad(P, X, y, pos_weight=2.5, scale=1.0 / n)
+    print("loss gpu", loss, "restatement", lr)
+    assert abs(loss - lr) <= 1e-5 * abs(lr)
+    groups = {"W1": (M.OFF_W1, M.OFF_B1), "b1": (M.OFF_B1, M.OFF_W2), "W2": (M.OFF_W2, M.OFF_B2),
+              "b2": (M.OFF_B2, M.OFF_W3), "w3": (M.OFF_W3, M.OFF_B3), "b3": (M.OFF_B3, M.NPARAM)}
+    for name, (a, b) in groups.items():
+        err = np.linalg.norm(G[a:b] - Gr[a:b]) / np.linalg.norm(Gr[a:b])
+        print(f"grad {name}: rel L2 {err:.3e}")
+        assert err <= 5e-3, name
+
+
+def test_mlp_adam_matches_restatement(require_gpu):
+    torch = _torch()
+    rng = np.random.default_rng(3)
+    P = _params(3)
+    Mm = rng.normal(0, 1e-3, M.NPARAM).astype(np.float32)
+    V = rng.uniform(0, 1e-5, M.NPARAM).astype(np.float32)
+    G = rng.normal(0, 1e-2, M.NPARAM).astype(np.float32)
+    dev = [torch.from_numpy(a.copy()).cuda() for a in (P, Mm, V, G)]
+    H.mlp_adam(*dev, t=5, lr=1e-3)
+    M.adam(P, Mm, V, G, 5, 1e-3)
+    for a, b in zip(dev[:3], (P, Mm, V)):
+        np.testing.assert_allclose(a.cpu().numpy(), b, rtol=2e-6, atol=1e-9)
+
+
+def _setup(problem, **kw):
+    kw.setdefault("nt", 1)
+    kw.setdefault("tend", 1.0)
+    run = H.parse(H.make_input(problem=problem, **kw))
+    return run, H.init_problem(run), H.gpu_params(run)
+
+
+@pytest.mark.parametrize("problem,kw,steps", [
+    ("orszag_tang", dict(nx=32, ny=32, nz=1, cour=0.8), 4),
+    ("mhd_blast3d", dict(nx=12, ny=14, nz=16, x1bc="open", cour=0.4), 2),
+])
+def test_collect_samples_match_restatement(problem, kw, steps, require_gpu):
+    torch = _torch()
+    run, q0, p = _setup(problem, **kw)
+    print(f"[collect] {problem} {kw}", flush=True)
+    g = H.GpuSolver(p)
+    g.upload_q0(q0)
+    g.init_faces()
+    for _ in range(steps):
+        g.ssprk(g.tstep())
+    qa = H.active(g.q0())
+    for d in range(3):
+        n = g.n_samples(d)
+        if n == 0:
+            continue
+        X = torch.zeros((n, 18), dtype=torch.float32, device="cuda")
+        y = torch.zeros(n, dtype=torch.float32, device="cuda")
+        assert g.collect_samples(d, X, y) == n
+        X, y = X.cpu().numpy(), y.cpu().numpy()
+        per = p.bc[d] == H.BC["periodic"]
+        fr = M.interface_features(qa, d, per, p.gam, p.rhomin, p.pgmin)
+        lr, dev = M.interface_labels(qa, d, per, p.gam, O.line_maxdev, p.rhomin, p.pgmin)
+        assert X.shape == fr.shape
+        np.testing.assert_allclose(X, fr, atol=2e-5)
+        clear = np.abs(dev - 0.05) > 1e-9
+        print(f"dir {d}: {n} samples, flagged fraction {y.mean():.3f}")
+        assert np.array_equal(y[clear], lr[clear])
+        assert 0 < y.sum() < n
+
+
+def _pair(problem, kw, exact):
+    run, q0, p = _setup(problem, **kw)
+    gp = H.GpuSolver(p, exact=exact)
+    return run, q0, p, gp
+
+
+@pytest.mark.parametrize("problem,kw", [
+    ("orszag_tang", dict(nx=32, ny=32, nz=1, cour=0.8)),
+    ("mhd_blast3d", dict(nx=12, ny=14, nz=16, x1bc="open", cour=0.4)),
+])
+def test_routing_extremes_exact_build(problem, kw, require_gpu):
+    run, q0, p = _setup(problem, **kw)
+    print(f"[routing extremes] {problem} {kw}", flush=True)
+    ref_w, ref_l = O.RefSolver(p), O.RefSolver(p)
+    ref_l.set_route(1)
+    g_all, g_none = H.GpuSolver(p, exact=True), H.GpuSolver(p, exact=True)
+    g_all.set_indicator(_params(0, b3=50.0), 0.5, enable=2)
+    g_none.set_indicator(_params(0, b3=-50.0), 0.5, enable=2)
+    for s in (ref_w, ref_l, g_all, g_none):
+        s.upload_q0(q0)
+        s.init_faces()
+    for _ in range(3):
+        dt = ref_w.tstep()
+        for s in (ref_w, ref_l, g_all, g_none):
+            s.ssprk(dt)
+    for g, r in ((g_all, ref_w), (g_none, ref_l)):
+        a, b = H.active(g.q0()), H.active(r.q0())
+        assert np.count_nonzero(a.view(np.int64) != b.view(np.int64)) == 0
+        fa, fb = H.active(g.faces()), H.active(r.faces())
+        assert np.count_nonzero(fa.view(np.int64) != fb.view(np.int64)) == 0
+    st_all, st_none = g_all.indicator_stats(), g_none.indicator_stats()
+    assert st_all["flagged"] == st_all["total"] > 0
+    assert st_none["flagged"] == 0 and st_none["total"] == st_all["total"]
+    # interfaces evaluated: 15 sweeps (5 stages x 3 steps) over every direction
+    per_step = sum(g_all.n_samples(d) for d in range(3))
+    assert st_all["total"] == 15 * per_step
+
+
+def test_routing_mixed_conserves(require_gpu):
+    run, q0, p = _setup("mpi_smooth3d", nx=32, ny=24, nz=16, cour=0.8)
+    g = H.GpuSolver(p)
+    # b3 = 0 with random hidden layers: a mixed routing
+    P = _params(5)
+    P[M.OFF_B3] = 0.0
+    g.set_indicator(P, 0.5, enable=2)
+    g.upload_q0(q0)
+    g.init_faces()
+    d0 = g.diag()
+    b0 = g.faces()
+    for _ in range(5):
+        g.ssprk(g.tstep())
+    st = g.indicator_stats()
+    print("mixed routing stats:", st)
+    assert 0.02 < st["flagged_fraction"] < 0.98
+    d1 = g.diag()
+    a0 = np.abs(H.active(q0)).reshape(8, -1).sum(axis=1)
+    for m in (0, 1, 2, 3, 7):
+        scale = a0[1:4].sum() if m in (1, 2, 3) else a0[m]
+        assert abs(d1.sum_cons[m] - d0.sum_cons[m]) <= 1e-12 * scale, m
+    aa, bb = 9.0 / 8.0, -1.0 / 24.0
+
+    def div(b):
+        act = H.active(b)
+        out = 0.0
+        for dd, h in enumerate((p.dx, p.dy, p.dz)):
+            f = act[dd]
+            out = out + (aa * (f - np.roll(f, 1, dd)) + bb * (np.roll(f, -1, dd) - np.roll(f, 2, dd))) / h
+        return out
+    b1 = g.faces()
+    scale = np.abs(H.active(b1)).max() / min(p.dx, p.dy, p.dz)
+    assert np.abs(div(b1) - div(b0)).max() <= 1e-13 * scale
+
+
+def test_routing_fp32_all_flagged_matches_plain_fp32(require_gpu):
+    run, q0, p = _setup("mhd_blast3d", nx=32, ny=32, nz=32, cour=0.4)
+    p.precision = H.PREC["fp32"]
+    a, b = H.GpuSolver(p), H.GpuSolver(p)
+    b.set_indicator(_params(0, b3=50.0), 0.5, enable=1)
+    for s in (a, b):
+        s.upload_q0(q0)
+        s.init_faces()
+    for _ in range(3):
+        dt = a.tstep()
+        a.ssprk(dt)
+        b.ssprk(dt)
+    l1 = H.rel_l1(b.q0(), a.q0())
+    print("fp32 routed(all) vs plain fp32 rel L1:", l1)
+    assert np.all(l1 <= 1e-12)
+
+
+def test_indicator_mode_requires_weights(require_gpu):
+    run, q0, p = _setup("orszag_tang", nx=16, ny=16, nz=1)
+    p.indicator = H.INDICATOR["mlp"]
+    g = H.GpuSolver(p)
+    g.upload_q0(q0)
+    g.init_faces()
+    with pytest.raises(H.HowmhdError, match="needs weights"):
+        g.ssprk(1e-3)
+    with pytest.raises(H.HowmhdError, match="threshold"):
+        g.set_indicator(_params(0), 1.5)
+    g.set_indicator(_params(0), 0.5)
+    g.ssprk(g.tstep())

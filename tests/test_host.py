@@ -27,6 +27,7 @@ def declared_functions(header: Path):
 @pytest.mark.parametrize("header,libs", [
     ("howmhd_gpu.h", ["libhowmhd_gpu.so", "libhowmhd_gpu_exact.so"]),
     ("howmhd_host.h", ["libhowmhd_host.so"]),
+    ("howmhd_mlp.h", ["libhowmhd_gpu.so", "libhowmhd_gpu_exact.so"]),
 ])
 def test_every_declared_symbol_is_exported(header, libs):
     import ctypes
@@ -248,3 +249,26 @@ def test_output_none_and_unknown(tmp_path):
     r.output_format = b"hdf5"
     with pytest.raises(H.HowmhdError, match="unknown output_format = hdf5"):
         H.write_output(r, q, 0, 0.0, str(tmp_path))
+
+
+def test_indicator_weights_roundtrip(tmp_path):
+    import train_indicator as T
+    rng = np.random.default_rng(0)
+    P = rng.normal(size=H.MLP_NPARAM).astype(np.float32)
+    path = tmp_path / "w.txt"
+    T.write_weights(path, P, 0.37, {"note": "test"})
+    Q, thr = H.load_indicator(path)
+    np.testing.assert_array_equal(P, Q)
+    assert abs(thr - 0.37) < 1e-7
+    Q2, thr2 = T.read_weights(path)
+    np.testing.assert_array_equal(P, Q2)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("howmhd-mlp 1697 0.5\n1.0\n")
+    with pytest.raises(H.HowmhdError, match="1 of 1697"):
+        H.load_indicator(bad)
+    bad.write_text("howmhd-mlp 12 0.5\n")
+    with pytest.raises(H.HowmhdError, match="expected 1697"):
+        H.load_indicator(bad)
+    bad.write_text("howmhd-mlp 1697 1.5\n")
+    with pytest.raises(H.HowmhdError, match="threshold"):
+        H.load_indicator(bad)
