@@ -99,7 +99,13 @@ __device__ __forceinline__ void layer_relu(const uint16_t* src, uint16_t* dst, c
     }
 }
 
-// Features of one interface from its 6-cell stencils of rho, p and |B|.
+// Features of one interface from its 6-cell stencils of rho, p and |B|: each
+// mapped to [0, 1] by its min and max, or all 0 when max - min <= MLP_FLAT
+// max(|min|, |max|).  The relative floor keeps round-off-level wiggles of
+// flat data (which leave the WENO weights ideal) from being stretched into
+// 0/1 patterns that look like jumps.
+constexpr float MLP_FLAT = 1e-5f;
+
 __device__ __forceinline__ void mlp_features(const float* rho, const float* p, const float* b, float* f) {
     const float* src[3] = {rho, p, b};
 #pragma unroll
@@ -111,7 +117,7 @@ __device__ __forceinline__ void mlp_features(const float* rho, const float* p, c
             hi = fmaxf(hi, src[g][k]);
         }
         const float span = hi - lo;
-        const float inv = span > 1e-12f * fmaxf(fabsf(hi), fabsf(lo)) + 1e-30f ? 1.0f / span : 0.0f;
+        const float inv = span > MLP_FLAT * fmaxf(fabsf(hi), fabsf(lo)) + 1e-30f ? 1.0f / span : 0.0f;
 #pragma unroll
         for (int k = 0; k < 6; ++k) f[6 * g + k] = (src[g][k] - lo) * inv;
     }

@@ -91,7 +91,8 @@ def _load(name: str) -> C.CDLL:
         path = LIB_DIR / name
         if not path.exists():
             raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (make -C how-mhd-kokkos_amd)")
-        _libs[name] = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+        # RTLD_LOCAL: the product and FP-exact builds export the same names
+        _libs[name] = C.CDLL(str(path), mode=C.RTLD_LOCAL)
     return _libs[name]
 
 

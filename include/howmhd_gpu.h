@@ -21,6 +21,12 @@
 
 #include <stdint.h>
 
+/* Only the functions declared in these headers are exported; the libraries
+ * are built with hidden visibility and bind their own symbols (-Bsymbolic),
+ * so the product build and its FP-contraction-off twin can be loaded into
+ * one process side by side. */
+#define HOWMHD_API __attribute__((visibility("default")))
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -84,59 +90,59 @@ typedef struct howmhd_gpu_ctx howmhd_gpu_ctx;
  * scratch on the current HIP device.  Replaces the allocation of q
  * (src/main.cpp:114-120) and allocate_bfield_ct (src/fluxct.cpp:27-35).
  * Axes must have n == 1 (degenerate) or n >= 3. */
-int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out);
+HOWMHD_API int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out);
 
 /* Frees everything.  Replaces deallocate_bfield_ct (src/fluxct.cpp:37-42). */
-int howmhd_gpu_destroy(howmhd_gpu_ctx* ctx);
+HOWMHD_API int howmhd_gpu_destroy(howmhd_gpu_ctx* ctx);
 
 /* Copies stage-0 q from the host (reference layout, ghosts ignored).  The
  * reference's problem initialisers write q(0,...) in place
  * (src/problem.cpp:39-59); this is the device hand-over of that array. */
-int howmhd_gpu_upload_q0(howmhd_gpu_ctx* ctx, const double* q0);
+HOWMHD_API int howmhd_gpu_upload_q0(howmhd_gpu_ctx* ctx, const double* q0);
 
 /* bound(q,0) + initialize_bfield_from_q + bound_bfield(0)
  * (src/main.cpp:127-130, src/fluxct.cpp:65-107). */
-int howmhd_gpu_init_faces(howmhd_gpu_ctx* ctx);
+HOWMHD_API int howmhd_gpu_init_faces(howmhd_gpu_ctx* ctx);
 
 /* Stage-0 faces from the host, 3 x (nx+6) x (ny+6) x (nz+6) (bxb, byb, bzb),
  * for restart or tests; ghosts ignored. */
-int howmhd_gpu_upload_faces(howmhd_gpu_ctx* ctx, const double* b);
+HOWMHD_API int howmhd_gpu_upload_faces(howmhd_gpu_ctx* ctx, const double* b);
 
 /* CFL step: dt = cour / sum_d max(vmax_d,1e-300)/Delta_d over active cells of
  * q(0) (src/tstep.cpp:11-136).  Synchronises. */
-int howmhd_gpu_tstep(howmhd_gpu_ctx* ctx, double* dt);
+HOWMHD_API int howmhd_gpu_tstep(howmhd_gpu_ctx* ctx, double* dt);
 
 /* One SSPRK(5,4) step of size dt (src/ssprk.cpp:1277-1329). */
-int howmhd_gpu_ssprk(howmhd_gpu_ctx* ctx, double dt);
+HOWMHD_API int howmhd_gpu_ssprk(howmhd_gpu_ctx* ctx, double dt);
 
 /* q(0) to the host in the reference layout, ghosts filled with the
  * periodic/open images (what src/bound.cpp leaves there).  Synchronises. */
-int howmhd_gpu_download_q0(howmhd_gpu_ctx* ctx, double* q0);
+HOWMHD_API int howmhd_gpu_download_q0(howmhd_gpu_ctx* ctx, double* q0);
 
 /* Stage-0 faces to the host, 3 x (nx+6) x (ny+6) x (nz+6): bxb, byb, bzb
  * (src/fluxct.cpp:10-12 at stage 0).  Synchronises. */
-int howmhd_gpu_download_faces(howmhd_gpu_ctx* ctx, double* b);
+HOWMHD_API int howmhd_gpu_download_faces(howmhd_gpu_ctx* ctx, double* b);
 
 /* Diagnostics of the stage-0 state.  Synchronises. */
-int howmhd_gpu_diag(howmhd_gpu_ctx* ctx, howmhd_diag* d);
+HOWMHD_API int howmhd_gpu_diag(howmhd_gpu_ctx* ctx, howmhd_diag* d);
 
 /* Starts (on != 0, counters reset) or stops per-kernel timing.  No
  * counterpart in the reference, which has no timers (src/ has no clock). */
-int howmhd_gpu_set_timing(howmhd_gpu_ctx* ctx, int on);
+HOWMHD_API int howmhd_gpu_set_timing(howmhd_gpu_ctx* ctx, int on);
 
 /* Resolves the timed launches into *t.  Synchronises. */
-int howmhd_gpu_get_timing(howmhd_gpu_ctx* ctx, howmhd_timing* t);
+HOWMHD_API int howmhd_gpu_get_timing(howmhd_gpu_ctx* ctx, howmhd_timing* t);
 
 /* Waits for all work queued on the context's stream. */
-int howmhd_gpu_synchronize(howmhd_gpu_ctx* ctx);
+HOWMHD_API int howmhd_gpu_synchronize(howmhd_gpu_ctx* ctx);
 
 /* Message of the last failing call on ctx (or of the last failing create
  * when ctx is NULL).  Never NULL. */
-const char* howmhd_gpu_last_error(const howmhd_gpu_ctx* ctx);
+HOWMHD_API const char* howmhd_gpu_last_error(const howmhd_gpu_ctx* ctx);
 
 /* Build identification: "fp-exact" for the library built with FP contraction
  * off (bitwise twin of the CPU restatement), "fp-fast" for the product build. */
-const char* howmhd_gpu_build_kind(void);
+HOWMHD_API const char* howmhd_gpu_build_kind(void);
 
 #ifdef __cplusplus
 }

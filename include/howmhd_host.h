@@ -39,30 +39,30 @@ typedef struct howmhd_run {
 /* read_parameters + finalize_parameters_after_read
  * (src/parameters.cpp:7-160).  `text` is the whole input file.  Returns 0, or
  * non-zero with a message in errbuf (the reference aborts instead). */
-int howmhd_host_parse(const char* text, howmhd_run* run, char* errbuf, int errlen);
+HOWMHD_API int howmhd_host_parse(const char* text, howmhd_run* run, char* errbuf, int errlen);
 
 /* Time-step parameters for howmhd_gpu_create. */
-int howmhd_host_gpu_params(const howmhd_run* run, howmhd_params* p, char* errbuf, int errlen);
+HOWMHD_API int howmhd_host_gpu_params(const howmhd_run* run, howmhd_params* p, char* errbuf, int errlen);
 
 /* init_problem (src/problem.cpp:39-59): fills the active cells of q0, an
  * 8 x (nx+6) x (ny+6) x (nz+6) array (k fastest) that the caller zeroed.
  * Unknown names fail and list the registry in errbuf. */
-int howmhd_host_init_problem(const howmhd_run* run, double* q0, char* errbuf, int errlen);
+HOWMHD_API int howmhd_host_init_problem(const howmhd_run* run, double* q0, char* errbuf, int errlen);
 
 /* Names of the registered problems, separated by '\n'. */
-const char* howmhd_host_problem_names(void);
+HOWMHD_API const char* howmhd_host_problem_names(void);
 
 /* output (src/output.cpp:555-571): writes output/dumpNNNNNN_rank000000.{dat,vts}
  * under `dir` from stage-0 q in the reference layout.  format "none" writes
  * nothing. */
-int howmhd_host_output(const howmhd_run* run, const double* q0, int step, double time,
+HOWMHD_API int howmhd_host_output(const howmhd_run* run, const double* q0, int step, double time,
                        const char* dir, char* errbuf, int errlen);
 
 /* Reads a learned-indicator weights file (the `indicator_weights` input key;
  * no counterpart in the reference): text, '#' lines are comments, then the
  * header "howmhd-mlp <nparam> <threshold>" and nparam FP32 values.  nparam
  * must equal HOWMHD_MLP_NPARAM (1697) and the threshold lie in (0, 1). */
-int howmhd_host_load_indicator(const char* path, float* params, int nparam, float* threshold,
+HOWMHD_API int howmhd_host_load_indicator(const char* path, float* params, int nparam, float* threshold,
                                char* errbuf, int errlen);
 
 #ifdef __cplusplus

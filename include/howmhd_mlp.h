@@ -9,7 +9,8 @@
  * Parameter vector (FP32 master copy, HOWMHD_MLP_NPARAM floats):
  *   W1[32][18], b1[32], W2[32][32], b2[32], w3[32], b3
  * Features (18 floats per interface): the 6-cell WENO stencils of rho, p and
- * |B|, each mapped to [0, 1] by its min and max over the 6 cells.
+ * |B|, each mapped to [0, 1] by its min and max over the 6 cells (all 0 when
+ * max - min <= 1e-5 max(|min|, |max|)).
  * Label: 1 when max over waves, sides and r of |w_r - d_r| > 0.05, with
  * d = (0.1, 0.6, 0.3).
  *
@@ -30,16 +31,16 @@ extern "C" {
 #define HOWMHD_MLP_NFEAT 18
 
 /* logits[i] = z3 of sample i (X: n x 18 [dev], params [dev], logits [dev]). */
-int howmhd_mlp_infer(const float* X, int64_t n, const float* params, float* logits, void* stream);
+HOWMHD_API int howmhd_mlp_infer(const float* X, int64_t n, const float* params, float* logits, void* stream);
 
 /* Adds the gradient of sum_i w_i * BCE(sigmoid(z3_i), y_i) * scale to grad
  * [dev] (w_i = pos_weight for y_i = 1, else 1) and that loss to *loss [dev].
  * y: n floats in {0, 1} [dev]. */
-int howmhd_mlp_grad(const float* X, const float* y, int64_t n, const float* params, float pos_weight,
+HOWMHD_API int howmhd_mlp_grad(const float* X, const float* y, int64_t n, const float* params, float pos_weight,
                     float scale, float* grad, float* loss, void* stream);
 
 /* One Adam step on params [dev] with moments m, v [dev] (t >= 1). */
-int howmhd_mlp_adam(float* params, float* m, float* v, const float* grad, int64_t t, float lr, float beta1,
+HOWMHD_API int howmhd_mlp_adam(float* params, float* m, float* v, const float* grad, int64_t t, float lr, float beta1,
                     float beta2, float eps, void* stream);
 
 /* Samples of the current stage-0 state: for every interface of the sweep
@@ -47,7 +48,7 @@ int howmhd_mlp_adam(float* params, float* m, float* v, const float* grad, int64_
  * features to X [dev] and the label to y [dev] (float 0/1), computed by the
  * FP64 characteristic WENO5 of the plain method.  *count receives the number
  * of samples written (host).  capacity is the number of rows X and y hold. */
-int howmhd_gpu_collect_samples(howmhd_gpu_ctx* ctx, int dir, float* X, float* y, int64_t capacity,
+HOWMHD_API int howmhd_gpu_collect_samples(howmhd_gpu_ctx* ctx, int dir, float* X, float* y, int64_t capacity,
                                int64_t* count);
 
 /* Loads indicator weights (params on the host, HOWMHD_MLP_NPARAM floats) and
@@ -56,13 +57,13 @@ int howmhd_gpu_collect_samples(howmhd_gpu_ctx* ctx, int dir, float* X, float* y,
  * (howmhd_gpu_indicator_stats), 0 turns it off (params may be NULL then).
  * A context created with indicator HOWMHD_INDICATOR_MLP refuses to step until
  * weights are loaded.  Synchronises. */
-int howmhd_gpu_set_indicator(howmhd_gpu_ctx* ctx, const float* params, float threshold, int enable);
+HOWMHD_API int howmhd_gpu_set_indicator(howmhd_gpu_ctx* ctx, const float* params, float threshold, int enable);
 
 /* Interfaces the indicator flagged / interfaces evaluated since the last
  * call (or set_indicator), and the same count at wavefront granularity (a
  * wavefront runs the full WENO5 when any of its interfaces is flagged).
  * Counted only with enable == 2.  Synchronises and resets the counters. */
-int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* ctx, int64_t* flagged, int64_t* total,
+HOWMHD_API int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* ctx, int64_t* flagged, int64_t* total,
                                int64_t* waves_flagged, int64_t* waves_total);
 
 #ifdef __cplusplus

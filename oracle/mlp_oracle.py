@@ -6,7 +6,7 @@ feature and label code), so the GPU results can be checked against it:
 
   features  for each of rho, p, |B| over the 6-cell WENO stencil (cells
             t-3 .. t+2 of interface t), in float32:
-            (x - min) / (max - min), or 0 when max - min <= 1e-12 max|x| + 1e-30
+            (x - min) / (max - min), or 0 when max - min <= 1e-5 max|x| + 1e-30
   forward   x, W1, W2, w3 rounded to bf16 (round to nearest even);
             z1 = x W1^T + b1; a1 = bf16(relu(z1)); z2 = a1 W2^T + b2;
             a2 = bf16(relu(z2)); z3 = b3 + a2 . w3
@@ -26,6 +26,7 @@ from __future__ import annotations
 import numpy as np
 
 NIN, NH = 18, 32
+FLAT = np.float32(1e-5)
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3, NPARAM = 0, 576, 608, 1632, 1664, 1696, 1697
 
 
@@ -113,15 +114,17 @@ def stencil_features(rho6, p6, b6) -> np.ndarray:
         g = np.asarray(g, dtype=np.float32)
         lo, hi = g.min(axis=1, keepdims=True), g.max(axis=1, keepdims=True)
         span = hi - lo
-        thr = np.float32(1e-12) * np.maximum(np.abs(hi), np.abs(lo)) + np.float32(1e-30)
+        thr = FLAT * np.maximum(np.abs(hi), np.abs(lo)) + np.float32(1e-30)
         inv = np.where(span > thr, np.float32(1) / np.where(span > 0, span, np.float32(1)), np.float32(0))
         out.append(((g - lo) * inv).astype(np.float32))
     return np.concatenate(out, axis=1)
 
 
-def cell_primitives(qa, gam, rhomin, pgmin):
-    """rho, p, |B| of the active cells (8, nx, ny, nz) with the floors of
-    src/ssprk.cpp:458-516."""
+def cell_primitives(qa, gam, rhomin, pgmin, d=0):
+    """rho, p (float64, with the floors of src/ssprk.cpp:458-516) and |B|
+    (float32) of the active cells (8, nx, ny, nz).  |B| is formed as the
+    sweep along axis d forms it: B_n^2 + B_t1^2 + B_t2^2 summed in float64 in
+    that order, rounded to float32, then a float32 square root."""
     rho = qa[0]
     v2 = (qa[1] ** 2 + qa[2] ** 2 + qa[3] ** 2) / rho ** 2
     BB2 = qa[4] ** 2 + qa[5] ** 2 + qa[6] ** 2
@@ -129,7 +132,9 @@ def cell_primitives(qa, gam, rhomin, pgmin):
     fl = (rho < rhomin) | (pg < pgmin)
     rho_f = np.where(fl, np.maximum(rho, rhomin), rho)
     pg_f = np.where(fl, np.maximum(pg, pgmin), pg)
-    return rho_f, pg_f, np.sqrt(BB2)
+    bn, b1, b2 = (qa[4 + (d + k) % 3] for k in range(3))
+    bmag = np.sqrt((bn * bn + b1 * b1 + b2 * b2).astype(np.float32))
+    return rho_f, pg_f, bmag
 
 
 def _axis_map(a, n, per):
@@ -142,7 +147,7 @@ def interface_features(qa, d, per, gam, rhomin=1e-12, pgmin=1e-12):
     """Features of every interface of the sweep along axis d (0 x, 1 y, 2 z) of
     the active state qa (8, nx, ny, nz), in the sample order of
     howmhd_gpu_collect_samples: x lines (z, y, t), y (z, t, x), z (t, y, x)."""
-    rho, pg, bm = (np.asarray(a, dtype=np.float32) for a in cell_primitives(qa, gam, rhomin, pgmin))
+    rho, pg, bm = (np.asarray(a, dtype=np.float32) for a in cell_primitives(qa, gam, rhomin, pgmin, d))
     n = qa.shape[1 + d]
     tbeg = 1 if per else 0
     ts = np.arange(tbeg, n + 1)

@@ -108,3 +108,26 @@ def test_orszag_tang_128_ten_steps(require_gpu):
     assert np.all(l1 <= 1e-12), l1
     d = gpu.diag()
     print("max |div B| (CT faces):", d.max_divb_ct)
+
+
+def test_builds_stay_distinct_in_one_process(require_gpu):
+    """Both GPU builds loaded into one process keep their own kernels: the
+    product build (loaded first here) runs with FP contraction and differs
+    from the restatement in the last bits, the FP-contraction-off build stays
+    bit-for-bit equal to it."""
+    run, q0, p = _setup("orszag_tang", dict(nx=32, ny=32, nz=1, cour=0.8))
+    fast = H.GpuSolver(p)
+    exact = H.GpuSolver(p, exact=True)
+    ref = O.RefSolver(p)
+    assert H.build_kind() == "fp-fast" and H.build_kind(exact=True) == "fp-exact"
+    for s in (fast, exact, ref):
+        s.upload_q0(q0)
+        s.init_faces()
+    for _ in range(2):
+        dt = ref.tstep()
+        for s in (fast, exact, ref):
+            s.ssprk(dt)
+    qr = ref.q0()
+    assert _bitwise_mismatch(exact.q0(), qr) == 0
+    assert _bitwise_mismatch(fast.q0(), qr) > 0
+    assert np.all(H.rel_l1(fast.q0(), qr) <= 1e-12)
