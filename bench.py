@@ -3,7 +3,8 @@
 
 Workload: 3-D MHD blast (problem mhd_blast3d) on a 256^3 periodic unit box,
 rho=1, p=0.1 ambient and p=10 for r<0.1, B=(1/sqrt2, 1/sqrt2, 0), v=0,
-gam=5/3, cour=0.4, FP64 plain method.  One "step" is one hydro cycle: the CFL
+gam=5/3, cour=0.4, plain method in FP64 with the reference's rounding (bit for
+bit equal to the CPU restatement).  One "step" is one hydro cycle: the CFL
 reduction (tstep) plus one SSPRK(5,4) step with its 15 directional sweeps,
 5 CT updates and 5 stage combinations.  q(0) and the faces are resident in
 HBM before the timed region; the per-step dt read-back is part of the loop,
@@ -57,7 +58,7 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "fp64fast"], default="fp64")
     ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -113,49 +114,66 @@ def cpu_baseline(n: int, steps: int):
     }
 
 
-def fp32_form(H, torch, run, args, fp64_value):
-    """The precision form (FP32 sweeps and CT EMFs, FP64 stage sums, E->p,
-    floors and face-B): zone-cycles/s on the same workload, its speedup over the
-    plain FP64 path, and its relative L1 error against the FP64 path after
-    --err-steps steps driven with the FP64 path's dt sequence."""
+WEIGHTS = ROOT / "how-mhd-kokkos_amd" / "weights" / "indicator_mlp.txt"
+
+
+def form_leg(H, torch, run, args, fp64_value, precision, indicator=False):
+    """One form of the time step on the same workload: zone-cycles/s, speedup
+    over the plain FP64 path, and relative L1 against the plain FP64 path
+    after --err-steps steps driven with the plain path's dt sequence.
+    precision: fp64fast (FMA rounding), fp32 (FP32 sweeps and CT EMFs; FP64
+    stage sums, E->p, floors and face-B); indicator: the learned smoothness
+    indicator routes unflagged interfaces to linear upwind-5."""
     import numpy as np
 
     n = args.size
     q0 = H.init_problem(run)
     p64 = H.gpu_params(run)
-    p32 = H.gpu_params(run)
-    p32.precision = H.PREC["fp32"]
-    g32 = H.GpuSolver(p32)
-    g32.upload_q0(q0)
-    g32.init_faces()
+    pf = H.gpu_params(run)
+    pf.precision = H.PREC[precision]
+    gf = H.GpuSolver(pf)
+    extra = {}
+    if indicator:
+        P, thr = H.load_indicator(WEIGHTS)
+        gf.set_indicator(P, thr, enable=1)
+        extra["indicator_weights"] = str(WEIGHTS.relative_to(ROOT))
+        extra["threshold"] = thr
+    gf.upload_q0(q0)
+    gf.init_faces()
     for _ in range(args.warmup):
-        g32.ssprk(g32.tstep())
-    g32.synchronize()
+        gf.ssprk(gf.tstep())
+    gf.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        g32.ssprk(g32.tstep())
-    g32.synchronize()
+        gf.ssprk(gf.tstep())
+    gf.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     value = n ** 3 * args.steps / el
-    # error against the FP64 path, same dt sequence, from the same q(0)
+    # error against the plain FP64 path, same dt sequence, from the same q(0)
+    if indicator:
+        gf.set_indicator(P, thr, enable=2)
     g64 = H.GpuSolver(p64)
-    for s in (g32, g64):
+    for s in (gf, g64):
         s.upload_q0(q0)
         s.init_faces()
     del q0
     for _ in range(args.err_steps):
         dt = g64.tstep()
         g64.ssprk(dt)
-        g32.ssprk(dt)
-    a32, a64 = g32.q0(), g64.q0()
-    l1 = H.rel_l1(a32, a64)
+        gf.ssprk(dt)
+    af, a64 = gf.q0(), g64.q0()
+    l1 = H.rel_l1(af, a64)
     emag = lambda a: 0.5 * float((H.active(a)[4:7] ** 2).sum())
-    d32 = g32.diag()
-    g32.close()
+    df = gf.diag()
+    if indicator:
+        st = gf.indicator_stats()
+        extra["flagged_fraction"] = round(st["flagged_fraction"], 4)
+        extra["wavefront_flagged_fraction"] = round(st["wave_fraction"], 4)
+    gf.close()
     g64.close()
-    return {
+    out = {
         "value": round(value, 1),
         "unit": "zone-cycles/s",
         "ms_per_step": round(el / args.steps * 1e3, 3),
@@ -164,9 +182,11 @@ def fp32_form(H, torch, run, args, fp64_value):
         "rel_l1_rho": float(l1[0]),
         "rel_l1_E": float(l1[7]),
         "rel_l1_max": float(np.max(l1)),
-        "magnetic_energy_rel_err": abs(emag(a32) - emag(a64)) / emag(a64),
-        "divb_ct_max": d32.max_divb_ct,
+        "magnetic_energy_rel_err": abs(emag(af) - emag(a64)) / emag(a64),
+        "divb_ct_max": df.max_divb_ct,
     }
+    out.update(extra)
+    return out
 
 
 def main():
@@ -264,10 +284,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 sweeps/CT, f64 stage sums",
+        "dtype": {"fp64": "f64", "fp64fast": "f64", "fp32": "f32 sweeps/CT, f64 stage sums"}[args.precision],
         "data": "synthetic: deterministic mhd_blast3d initial condition",
         "config": {"workload": f"mhd_blast3d {n}^3 periodic, gam 5/3, cour 0.4, p_in 10, p_out 0.1, "
-                               f"B=(1/sqrt2,1/sqrt2,0), {args.precision} plain method",
+                               f"B=(1/sqrt2,1/sqrt2,0), plain method, precision {args.precision}",
                    "grid": [n, n, n], "precision": args.precision, "parallelism": "single GPU"},
         "roofline": roof,
         "kernels": kernels,
@@ -275,7 +295,11 @@ def main():
         "divb_ct_max": d.max_divb_ct,
     }
     if not args.no_forms and args.precision == "fp64":
-        out["forms"] = {"fp32": fp32_form(H, torch, run, args, value)}
+        forms = {"fp64fast": form_leg(H, torch, run, args, value, "fp64fast"),
+                 "fp32": form_leg(H, torch, run, args, value, "fp32")}
+        if WEIGHTS.exists():
+            forms["fp32+indicator"] = form_leg(H, torch, run, args, value, "fp32", indicator=True)
+        out["forms"] = forms
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_size, args.cpu_steps)
     print(json.dumps(out), flush=True)

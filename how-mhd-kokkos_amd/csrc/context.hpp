@@ -1,0 +1,97 @@
+// Device context of one run and the host-side helpers shared by the
+// translation units of the GPU library.
+//
+// The library is built from translation units with different floating-point
+// contraction: the plain FP64 form (reference rounding, -ffp-contract=off,
+// HOWMHD_EXACT=1) and the FMA forms (fp64fast, fp32; -ffp-contract=fast,
+// HOWMHD_EXACT=0).  Each unit instantiates its own kernels (their template
+// argument EX equals the unit's HOWMHD_EXACT), and exposes its stage launcher
+// through the functions declared at the end of this header.
+#pragma once
+
+#include "howmhd_gpu.h"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+struct howmhd_gpu_ctx {
+    howmhd_params p{};
+    howmhd::Geom g{};
+    howmhd::Phys ph{};
+    howmhd::CtFlags ctf{};
+    size_t ncell = 0;
+    size_t rsize = 8;          // bytes of the sweep/CT real type
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* qbuf[5] = {};      // 8 x ncell each
+    double* fbuf[5] = {};      // 3 x ncell each
+    int qid[6] = {0, 1, 2, 3, 4, 1};  // stage -> buffer
+    void* dF[3] = {};
+    void* ct[6] = {};          // fsy fsz gsx gsz hsx hsy
+    void* O2[3] = {};
+    void* O[3] = {};
+    unsigned long long* dmax = nullptr;
+    // learned indicator: device weights, routing mode (0 off, 1 on, 2 on and
+    // counted), counter slots
+    howmhd::MlpDev* mlp = nullptr;
+    int ind_mode = 0;
+    unsigned long long* istats = nullptr;
+    bool faces_ready = false;
+    bool timing = false;
+    struct Timed {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Timed> timed;
+    double tms[HOWMHD_K_COUNT] = {};
+    int tcount[HOWMHD_K_COUNT] = {};
+    std::string err = "no error";
+};
+
+#define HCHECK(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
+            return HOWMHD_EHIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+namespace howmhd {
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+inline int fail(howmhd_gpu_ctx* c, int code, const std::string& msg) {
+    c->err = msg;
+    return code;
+}
+
+// Brackets one launch with events when timing is on.
+struct TimedLaunch {
+    howmhd_gpu_ctx* c;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(howmhd_gpu_ctx* ctx, int k) : c(ctx), kind(k) {
+        if (c->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, c->stream);
+    }
+    ~TimedLaunch() {
+        if (a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->timed.push_back({kind, a, b});
+        }
+    }
+};
+
+// One SSPRK stage (sweeps, EMFs, faces, combination) of each form.
+int stage_fp64(howmhd_gpu_ctx* c, int iw, double dt);      // step_exact.hip
+int stage_fp64fast(howmhd_gpu_ctx* c, int iw, double dt);  // step_fast.hip
+int stage_fp32(howmhd_gpu_ctx* c, int iw, double dt);      // step_fast.hip
+// Indicator samples of the stage-0 state along axis dir from the plain FP64
+// sweep (step_exact.hip).
+int collect_fp64(howmhd_gpu_ctx* c, int dir, float* X, float* y);
+
+}  // namespace howmhd

@@ -6,9 +6,24 @@
 // copy_stage(q,0,5)), the three directional interface-flux arrays, the six CT
 // flux arrays and the six EMF arrays.  All work is queued on the context's
 // stream; tstep and downloads synchronise.
+// C-ABI of the HIP time step (declared in include/howmhd_gpu.h and
+// include/howmhd_mlp.h).
+//
+// One context owns the device state of a run: five stage buffers of q and of
+// the faces (stage 5 reuses the buffer of stage 1, which is dead after stage
+// 2; the step ends by swapping it with stage 0 instead of the reference's
+// copy_stage(q,0,5)), the three directional interface-flux arrays, the six CT
+// flux arrays and the six EMF arrays.  All work is queued on the context's
+// stream; tstep and downloads synchronise.
+//
+// This unit is built with -ffp-contract=off -DHOWMHD_EXACT=1: the CFL
+// reduction and the face initialisation are the reference's bit for bit.  The
+// stage kernels live in step_exact.hip (plain FP64) and step_fast.hip
+// (fp64fast, fp32).
 #include "howmhd_gpu.h"
 #include "howmhd_mlp.h"
-#include "kernels.hpp"
+#include "context.hpp"
+#include "tstep_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -19,97 +34,15 @@
 #include <string>
 #include <vector>
 
+#if !HOWMHD_EXACT
+#error "howmhd_gpu.hip is built with HOWMHD_EXACT=1"
+#endif
+
 using namespace howmhd;
 
 namespace {
 
 std::string g_create_error = "no error";
-
-struct RK {
-    double k1, k2, k3, k4, k5;
-};
-
-// SSPRK(5,4) coefficients, src/ssprk.cpp:66-94
-RK rk_coeff(int iw) {
-    RK c{0.0, 0.0, 0.0, 0.0, 0.0};
-    switch (iw) {
-    case 1: c.k1 = 1.0; c.k2 = 0.39175222700392; c.k3 = 0.0; break;
-    case 2: c.k1 = 0.44437049406734; c.k2 = 0.36841059262959; c.k3 = 0.55562950593266; break;
-    case 3: c.k1 = 0.62010185138540; c.k2 = 0.25189177424738; c.k3 = 0.37989814861460; break;
-    case 4: c.k1 = 0.17807995410773; c.k2 = 0.54497475021237; c.k3 = 0.82192004589227; break;
-    case 5:
-        c.k1 = -2.081261929715610e-02; c.k2 = 0.22600748319395; c.k3 = 5.03580947213895e-1;
-        c.k4 = 0.51723167208978; c.k5 = -6.518979800418380e-12;
-        break;
-    }
-    return c;
-}
-
-}  // namespace
-
-struct howmhd_gpu_ctx {
-    howmhd_params p{};
-    Geom g{};
-    Phys ph{};
-    CtFlags ctf{};
-    size_t ncell = 0;
-    size_t rsize = 8;          // bytes of the sweep/CT real type
-    int device = 0;
-    hipStream_t stream = nullptr;
-    double* qbuf[5] = {};      // 8 x ncell each
-    double* fbuf[5] = {};      // 3 x ncell each
-    int qid[6] = {0, 1, 2, 3, 4, 1};  // stage -> buffer
-    void* dF[3] = {};
-    void* ct[6] = {};          // fsy fsz gsx gsz hsx hsy
-    void* O2[3] = {};
-    void* O[3] = {};
-    unsigned long long* dmax = nullptr;
-    // learned indicator: device weights, routing mode (0 off, 1 on, 2 on and
-    // counted), counter slots
-    MlpDev* mlp = nullptr;
-    int ind_mode = 0;
-    unsigned long long* istats = nullptr;
-    bool faces_ready = false;
-    bool timing = false;
-    struct Timed {
-        int kind;
-        hipEvent_t a, b;
-    };
-    std::vector<Timed> timed;
-    double tms[HOWMHD_K_COUNT] = {};
-    int tcount[HOWMHD_K_COUNT] = {};
-    std::string err = "no error";
-};
-
-#define HCHECK(ctx, call)                                                                  \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
-            return HOWMHD_EHIP;                                                            \
-        }                                                                                  \
-    } while (0)
-
-namespace {
-
-unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
-
-// Brackets one launch with events when timing is on.
-struct TimedLaunch {
-    howmhd_gpu_ctx* c;
-    int kind;
-    hipEvent_t a = nullptr, b = nullptr;
-    TimedLaunch(howmhd_gpu_ctx* ctx, int k) : c(ctx), kind(k) {
-        if (c->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
-            (void)hipEventRecord(a, c->stream);
-    }
-    ~TimedLaunch() {
-        if (a && b) {
-            (void)hipEventRecord(b, c->stream);
-            c->timed.push_back({kind, a, b});
-        }
-    }
-};
 
 void drain_timing(howmhd_gpu_ctx* c) {
     for (auto& t : c->timed) {
@@ -122,56 +55,6 @@ void drain_timing(howmhd_gpu_ctx* c) {
         (void)hipEventDestroy(t.b);
     }
     c->timed.clear();
-}
-
-int fail(howmhd_gpu_ctx* c, int code, const std::string& msg) {
-    c->err = msg;
-    return code;
-}
-
-template <typename R, int MODE>
-void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
-    const Geom& g = c->g;
-    const int nif = g.per[d] ? g.n[d] : g.n[d] + 1;
-    if (d == 0) {
-        dim3 grid(blocks_for(nif, SweepTile<0>::LANES), blocks_for(g.n[1], SweepTile<0>::ROWS), g.n[2]);
-        hipLaunchKernelGGL((k_sweep<R, 0, MODE>), grid, dim3(256), 0, c->stream, a);
-    } else if (d == 1) {
-        dim3 grid(blocks_for(g.n[0], SweepTile<1>::LANES), blocks_for(nif, SweepTile<1>::ROWS), g.n[2]);
-        hipLaunchKernelGGL((k_sweep<R, 1, MODE>), grid, dim3(256), 0, c->stream, a);
-    } else {
-        dim3 grid(blocks_for(g.n[0], SweepTile<2>::LANES), blocks_for(nif, SweepTile<2>::ROWS), g.n[1]);
-        hipLaunchKernelGGL((k_sweep<R, 2, MODE>), grid, dim3(256), 0, c->stream, a);
-    }
-}
-
-template <typename R>
-SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
-    SweepArgs<R> a{};
-    a.q = qsrc;
-    a.dF = (R*)c->dF[d];
-    // x: fsy, fsz; y: gsz, gsx; z: hsx, hsy
-    a.ct_a = (R*)c->ct[2 * d];
-    a.ct_b = (R*)c->ct[2 * d + 1];
-    if (d == 1) { a.ct_a = (R*)c->ct[3]; a.ct_b = (R*)c->ct[2]; }
-    a.g = c->g;
-    a.ph = c->ph;
-    a.mlp = c->mlp;
-    a.stats = (c->ind_mode == 2) ? c->istats : nullptr;
-    return a;
-}
-
-template <typename R>
-int launch_sweeps(howmhd_gpu_ctx* c, const double* qsrc) {
-    for (int d = 0; d < 3; ++d) {
-        if (c->g.n[d] <= 1) continue;
-        const SweepArgs<R> a = sweep_args<R>(c, d, qsrc);
-        TimedLaunch tl(c, HOWMHD_K_SWEEP_X + d);
-        if (c->ind_mode) launch_sweep<R, SW_ROUTED>(c, d, a);
-        else launch_sweep<R, SW_PLAIN>(c, d, a);
-        HCHECK(c, hipGetLastError());
-    }
-    return HOWMHD_OK;
 }
 
 // fp32 -> bf16 -> fp32, round to nearest even (the device's f2bf)
@@ -188,92 +71,6 @@ uint16_t bf16_bits(float f) {
     uint32_t u;
     std::memcpy(&u, &r, 4);
     return (uint16_t)(u >> 16);
-}
-
-template <typename R>
-int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
-    const RK k = rk_coeff(iw);
-    StageCoef s;
-    s.k1 = k.k1;
-    s.k3 = k.k3;
-    s.c4 = (iw == 5) ? k.k4 / k.k1 : 0.0;
-    s.c5 = (iw == 5) ? k.k5 / k.k1 : 0.0;
-    s.dtdx = k.k2 * dt / c->p.dx;   // src/ssprk.cpp:1302-1304
-    s.dtdy = k.k2 * dt / c->p.dy;
-    s.dtdz = k.k2 * dt / c->p.dz;
-    s.stage5 = (iw == 5);
-
-    const double* q0 = c->qbuf[c->qid[0]];
-    const double* qp = c->qbuf[c->qid[iw - 1]];
-    double* qo = c->qbuf[c->qid[iw]];
-    const double* b0 = c->fbuf[c->qid[0]];
-    const double* bp = c->fbuf[c->qid[iw - 1]];
-    double* bo = c->fbuf[c->qid[iw]];
-
-    int rc = launch_sweeps<R>(c, qp);
-    if (rc) return rc;
-
-    const unsigned nb = blocks_for(c->ncell, 256);
-    if (c->ctf.active) {
-        EmfArgs<R> e;
-        e.fsy = (const R*)c->ct[0]; e.fsz = (const R*)c->ct[1];
-        e.gsx = (const R*)c->ct[2]; e.gsz = (const R*)c->ct[3];
-        e.hsx = (const R*)c->ct[4]; e.hsy = (const R*)c->ct[5];
-        e.Ox2 = (R*)c->O2[0]; e.Oy2 = (R*)c->O2[1]; e.Oz2 = (R*)c->O2[2];
-        e.g = c->g;
-        e.f = c->ctf;
-        {
-            TimedLaunch tl(c, HOWMHD_K_EMF2);
-            hipLaunchKernelGGL((k_emf2<R>), dim3(nb), dim3(256), 0, c->stream, e);
-        }
-        HCHECK(c, hipGetLastError());
-        SmoothArgs<R> m;
-        m.Ox2 = (const R*)c->O2[0]; m.Oy2 = (const R*)c->O2[1]; m.Oz2 = (const R*)c->O2[2];
-        m.Ox = (R*)c->O[0]; m.Oy = (R*)c->O[1]; m.Oz = (R*)c->O[2];
-        m.g = c->g;
-        m.f = c->ctf;
-        {
-            TimedLaunch tl(c, HOWMHD_K_EMF);
-            hipLaunchKernelGGL((k_emf<R>), dim3(nb), dim3(256), 0, c->stream, m);
-        }
-        HCHECK(c, hipGetLastError());
-    }
-    FaceArgs<R> fa;
-    fa.Ox = (const R*)c->O[0]; fa.Oy = (const R*)c->O[1]; fa.Oz = (const R*)c->O[2];
-    fa.b0 = b0;
-    fa.b2 = c->fbuf[c->qid[2]];
-    fa.b3 = c->fbuf[c->qid[3]];
-    fa.bp = bp;
-    fa.bo = bo;
-    fa.g = c->g;
-    fa.f = c->ctf;
-    fa.s = s;
-    {
-        TimedLaunch tl(c, HOWMHD_K_FACES);
-        hipLaunchKernelGGL((k_faces<R>), dim3(nb), dim3(256), 0, c->stream, fa);
-    }
-    HCHECK(c, hipGetLastError());
-
-    CombineArgs<R> ca;
-    ca.q0 = q0;
-    ca.q2 = c->qbuf[c->qid[2]];
-    ca.q3 = c->qbuf[c->qid[3]];
-    ca.qp = qp;
-    ca.qo = qo;
-    ca.bo = bo;
-    ca.dFx = (const R*)c->dF[0];
-    ca.dFy = (const R*)c->dF[1];
-    ca.dFz = (const R*)c->dF[2];
-    ca.g = c->g;
-    ca.f = c->ctf;
-    ca.s = s;
-    ca.ph = c->ph;
-    {
-        TimedLaunch tl(c, HOWMHD_K_COMBINE);
-        hipLaunchKernelGGL((k_combine<R>), dim3(nb), dim3(256), 0, c->stream, ca);
-    }
-    HCHECK(c, hipGetLastError());
-    return HOWMHD_OK;
 }
 
 // reference layout (m, i, j, k) with ghosts <-> device layout (m, z, y, x) active
@@ -332,7 +129,8 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
             return HOWMHD_EINVAL;
         }
     }
-    if (p->precision != HOWMHD_PREC_FP64 && p->precision != HOWMHD_PREC_FP32) {
+    if (p->precision != HOWMHD_PREC_FP64 && p->precision != HOWMHD_PREC_FP32 &&
+        p->precision != HOWMHD_PREC_FP64_FAST) {
         g_create_error = "howmhd_gpu_create: unknown precision";
         return HOWMHD_EINVAL;
     }
@@ -484,8 +282,11 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
     if (!c->faces_ready) return fail(c, HOWMHD_ESTATE, "howmhd_gpu_ssprk: call init_faces or upload_faces first");
     if (c->p.indicator == HOWMHD_INDICATOR_MLP && !c->ind_mode)
         return fail(c, HOWMHD_ESTATE, "howmhd_gpu_ssprk: indicator mlp needs weights (howmhd_gpu_set_indicator)");
+    auto stage = (c->p.precision == HOWMHD_PREC_FP32)        ? &stage_fp32
+                 : (c->p.precision == HOWMHD_PREC_FP64_FAST) ? &stage_fp64fast
+                                                             : &stage_fp64;
     for (int iw = 1; iw <= 5; ++iw) {
-        const int rc = (c->rsize == sizeof(float)) ? launch_stage<float>(c, iw, dt) : launch_stage<double>(c, iw, dt);
+        const int rc = stage(c, iw, dt);
         if (rc) return rc;
     }
     // q(0) <- q(5), faces likewise: exchange the buffers of stages 0 and 5
@@ -617,16 +418,12 @@ int howmhd_gpu_collect_samples(howmhd_gpu_ctx* c, int dir, float* X, float* y, i
     if (capacity < need)
         return fail(c, HOWMHD_EINVAL, "howmhd_gpu_collect_samples: capacity " + std::to_string(capacity) +
                                           " < " + std::to_string(need) + " interfaces");
-    if (c->rsize != sizeof(double))
-        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_collect_samples: labels come from the FP64 solver (precision fp64)");
+    if (c->p.precision != HOWMHD_PREC_FP64)
+        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_collect_samples: labels come from the plain FP64 solver (precision fp64)");
     // the sweep writes this direction's dF and CT terms too; the next stage
     // recomputes them from its own source
-    SweepArgs<double> a = sweep_args<double>(c, dir, c->qbuf[c->qid[0]]);
-    a.feat = X;
-    a.label = y;
-    a.stats = nullptr;
-    launch_sweep<double, SW_COLLECT>(c, dir, a);
-    HCHECK(c, hipGetLastError());
+    const int rc = collect_fp64(c, dir, X, y);
+    if (rc) return rc;
     HCHECK(c, hipStreamSynchronize(c->stream));
     *count = need;
     return HOWMHD_OK;
@@ -682,11 +479,7 @@ int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* c, int64_t* flagged, int64_t* tot
 }
 
 const char* howmhd_gpu_build_kind(void) {
-#if HOWMHD_EXACT
-    return "fp-exact";
-#else
-    return "fp-fast";
-#endif
+    return "gfx950: fp64 reference rounding; fp64fast and fp32 with FMA contraction";
 }
 
 }  // extern "C"

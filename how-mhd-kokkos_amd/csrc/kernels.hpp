@@ -17,7 +17,10 @@
 //   k_combine       stage combination k1*q_init + k3*q_prev + dX + dY + dZ,
 //                   face->centre B (src/fluxct.cpp:882-897) and the floors of
 //                   prot (src/prot.cpp:10-51)
-// and per step k_cfl (src/tstep.cpp:28-83).
+// Each kernel carries EX = HOWMHD_EXACT of the translation unit that
+// instantiates it (the plain FP64 unit: reference rounding; the FMA unit:
+// fp64fast and fp32), so the two units never share a kernel symbol.  The
+// per-step kernels (CFL, face initialisation) are in tstep_kernels.hpp.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -240,8 +243,9 @@ __device__ __forceinline__ void r_col_dyn(int m, R gam, const Eig<R>& e, R* Rc) 
     }
 }
 
-template <typename R, int DIR, int MODE>
+template <typename R, int DIR, int MODE, int EX>
 __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
     using T = SweepTile<DIR>;
     __shared__ UF<R> luf[7][T::CELLS];
     __shared__ R lo[O_COUNT][T::CELLS];
@@ -526,8 +530,9 @@ __device__ __forceinline__ R i4(R a, R b, R c, R d) {
 }
 
 // fluxct_build_O2, src/fluxct.cpp:809-821, on active cells
-template <typename R>
+template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_emf2(EmfArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
     if (i >= g.ncell) return;
@@ -569,8 +574,9 @@ struct SmoothArgs {
 };
 
 // fluxct_smooth_O, src/fluxct.cpp:828-840
-template <typename R>
+template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_emf(SmoothArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
     if (i >= g.ncell) return;
@@ -622,8 +628,9 @@ __device__ __forceinline__ double d4(double om3, double om2, double om1, double 
 
 // fluxct_update_bfield, src/fluxct.cpp:850-877, with the stage initialisation
 // of copy_bfield_stage / prepare_bfield_stage5 (111-135) folded in.
-template <typename R>
+template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
     if (i >= g.ncell) return;
@@ -704,8 +711,9 @@ __device__ __forceinline__ double i6(double a, double b, double c, double d, dou
 // Stage combination: x_sweep's store q = k1*q + k3*q_prev + dX
 // (src/ssprk.cpp:415-430), then += dY, += dZ (1075-1094, 1166-1185); face ->
 // centre B (fluxct.cpp:882-897); prot (prot.cpp:10-51).
-template <typename R>
+template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
     if (i >= g.ncell) return;
@@ -797,75 +805,6 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     v[0] = rho;
 #pragma unroll
     for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
-}
-
-// ---------------------------------------------------------------- CFL (tstep)
-// Per-direction maximum of |v_d| + c_fast,d over active cells of q(0)
-// (src/tstep.cpp:28-83).  Non-negative doubles order like their bit patterns,
-// so the block maxima are combined with a 64-bit atomicMax.
-__global__ __launch_bounds__(256) void k_cfl(const double* q, Geom g, Phys ph, unsigned long long* out) {
-    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    const double gam = ph.gam;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < g.ncell; i += (long long)gridDim.x * 256) {
-        const double DD = q[i], Mx = q[g.ncell + i], My = q[2 * g.ncell + i], Mz = q[3 * g.ncell + i];
-        const double Bx = q[4 * g.ncell + i], By = q[5 * g.ncell + i], Bz = q[6 * g.ncell + i], EE = q[7 * g.ncell + i];
-        const double rho = fmax(DD, ph.rhomin);
-        const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
-        const double vv2 = vx * vx + vy * vy + vz * vz;
-        const double BB2 = Bx * Bx + By * By + Bz * Bz;
-        const double pg = fmax(ph.pgmin, (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2)));
-        const double bbn2 = BB2 / rho;
-        const double cs2 = fmax(0.0, gam * pg / rho);
-        const double bnx2 = Bx * Bx / rho, bny2 = By * By / rho, bnz2 = Bz * Bz / rho;
-        const double rootx = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
-        const double rooty = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bny2 * cs2));
-        const double rootz = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnz2 * cs2));
-        const double lfx = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootx)));
-        const double lfy = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rooty)));
-        const double lfz = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootz)));
-        m0 = fmax(m0, fabs(vx) + lfx);
-        m1 = fmax(m1, fabs(vy) + lfy);
-        m2 = fmax(m2, fabs(vz) + lfz);
-    }
-    // wavefront reduction (64 lanes), then across the 4 wavefronts
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        m0 = fmax(m0, __shfl_xor(m0, off, 64));
-        m1 = fmax(m1, __shfl_xor(m1, off, 64));
-        m2 = fmax(m2, __shfl_xor(m2, off, 64));
-    }
-    __shared__ double red[3][4];
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { red[0][w] = m0; red[1][w] = m1; red[2][w] = m2; }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const int d = threadIdx.x;
-        const double r = fmax(fmax(red[d][0], red[d][1]), fmax(red[d][2], red[d][3]));
-        atomicMax(&out[d], (unsigned long long)__double_as_longlong(r));
-    }
-}
-
-__global__ void k_zero_u64(unsigned long long* p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
-}
-
-// initialize_bfield_from_q, src/fluxct.cpp:65-107, on active faces
-__global__ __launch_bounds__(256) void k_init_faces(const double* q, double* b, Geom g) {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= g.ncell) return;
-    int x, y, z;
-    cell_of(g, i, x, y, z);
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const double* Bx = q + 4 * g.ncell;
-    const double* By = q + 5 * g.ncell;
-    const double* Bz = q + 6 * g.ncell;
-    const int px = g.per[0], py = g.per[1], pz = g.per[2];
-    b[i] = (-Bx[lin(g, bmap(x - 1, nx, px), y, z)] + 9.0 * Bx[i] + 9.0 * Bx[lin(g, bmap(x + 1, nx, px), y, z)] -
-            Bx[lin(g, bmap(x + 2, nx, px), y, z)]) / 16.0;
-    b[g.ncell + i] = (-By[lin(g, x, bmap(y - 1, ny, py), z)] + 9.0 * By[i] + 9.0 * By[lin(g, x, bmap(y + 1, ny, py), z)] -
-                      By[lin(g, x, bmap(y + 2, ny, py), z)]) / 16.0;
-    b[2 * g.ncell + i] = (-Bz[lin(g, x, y, bmap(z - 1, nz, pz))] + 9.0 * Bz[i] + 9.0 * Bz[lin(g, x, y, bmap(z + 1, nz, pz))] -
-                          Bz[lin(g, x, y, bmap(z + 2, nz, pz))]) / 16.0;
 }
 
 }  // namespace howmhd

@@ -143,8 +143,9 @@ extern "C" int howmhd_host_gpu_params(const howmhd_run* r, howmhd_params* p, cha
     }
     if (std::strcmp(r->precision, "fp64") == 0) p->precision = HOWMHD_PREC_FP64;
     else if (std::strcmp(r->precision, "fp32") == 0) p->precision = HOWMHD_PREC_FP32;
+    else if (std::strcmp(r->precision, "fp64fast") == 0) p->precision = HOWMHD_PREC_FP64_FAST;
     else {
-        fail(errbuf, errlen, std::string("Error: unknown precision = ") + r->precision + " (fp64, fp32)");
+        fail(errbuf, errlen, std::string("Error: unknown precision = ") + r->precision + " (fp64, fp32, fp64fast)");
         return HOWMHD_EINVAL;
     }
     if (std::strcmp(r->indicator, "off") == 0) p->indicator = HOWMHD_INDICATOR_OFF;

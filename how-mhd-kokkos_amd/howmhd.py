@@ -23,7 +23,7 @@ BIN_DIR = PKG_DIR / "bin"
 DRIVER = BIN_DIR / "how_mhd"
 
 BC = {"periodic": 0, "open": 1}
-PREC = {"fp64": 0, "fp32": 1}
+PREC = {"fp64": 0, "fp32": 1, "fp64fast": 2}
 INDICATOR = {"off": 0, "mlp": 1}
 
 
@@ -91,7 +91,7 @@ def _load(name: str) -> C.CDLL:
         path = LIB_DIR / name
         if not path.exists():
             raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (make -C how-mhd-kokkos_amd)")
-        # RTLD_LOCAL: the product and FP-exact builds export the same names
+        # RTLD_LOCAL: no other library binds to this one's symbols
         _libs[name] = C.CDLL(str(path), mode=C.RTLD_LOCAL)
     return _libs[name]
 
@@ -146,9 +146,9 @@ def _bind(lib: C.CDLL, table: dict) -> C.CDLL:
     return lib
 
 
-def gpu_lib(exact: bool = False) -> C.CDLL:
-    """The product library (fp-fast) or its FP-contraction-off twin (fp-exact)."""
-    return _bind(_load("libhowmhd_gpu_exact.so" if exact else "libhowmhd_gpu.so"), GPU_FUNCS)
+def gpu_lib() -> C.CDLL:
+    """The time-step library (every precision form)."""
+    return _bind(_load("libhowmhd_gpu.so"), GPU_FUNCS)
 
 
 def host_lib() -> C.CDLL:
@@ -221,8 +221,8 @@ def make_input(**kw) -> str:
 class GpuSolver:
     """One device context of the C-ABI: q(0) and faces stay resident."""
 
-    def __init__(self, params: Params, exact: bool = False):
-        self.lib = gpu_lib(exact)
+    def __init__(self, params: Params):
+        self.lib = gpu_lib()
         self.p = params
         self.shape = (params.nx + 6, params.ny + 6, params.nz + 6)
         h = C.c_void_p()
@@ -356,20 +356,20 @@ def _stream():
     return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
 
 
-def mlp_infer(X, P, exact: bool = False):
+def mlp_infer(X, P):
     """Logits z3 of the samples X (n, 18) with parameters P (1697,), both
     float32 CUDA tensors; runs on torch's current stream."""
     torch = _torch()
     _check_dev(X, (None, MLP_NFEAT))
     _check_dev(P, (MLP_NPARAM,))
     out = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
-    rc = gpu_lib(exact).howmhd_mlp_infer(X.data_ptr(), X.shape[0], P.data_ptr(), out.data_ptr(), _stream())
+    rc = gpu_lib().howmhd_mlp_infer(X.data_ptr(), X.shape[0], P.data_ptr(), out.data_ptr(), _stream())
     if rc:
         raise HowmhdError(f"howmhd_mlp_infer failed ({rc})")
     return out
 
 
-def mlp_grad(X, y, P, pos_weight=1.0, scale=1.0, G=None, loss=None, exact: bool = False):
+def mlp_grad(X, y, P, pos_weight=1.0, scale=1.0, G=None, loss=None):
     """Adds the gradient (and loss) of the weighted BCE over (X, y) to G
     (and loss); zero-initialised tensors are made when not given."""
     torch = _torch()
@@ -380,24 +380,24 @@ def mlp_grad(X, y, P, pos_weight=1.0, scale=1.0, G=None, loss=None, exact: bool 
         G = torch.zeros(MLP_NPARAM, dtype=torch.float32, device=X.device)
     if loss is None:
         loss = torch.zeros(1, dtype=torch.float32, device=X.device)
-    rc = gpu_lib(exact).howmhd_mlp_grad(X.data_ptr(), y.data_ptr(), X.shape[0], P.data_ptr(), pos_weight, scale,
+    rc = gpu_lib().howmhd_mlp_grad(X.data_ptr(), y.data_ptr(), X.shape[0], P.data_ptr(), pos_weight, scale,
                                         G.data_ptr(), loss.data_ptr(), _stream())
     if rc:
         raise HowmhdError(f"howmhd_mlp_grad failed ({rc})")
     return G, loss
 
 
-def mlp_adam(P, M, V, G, t: int, lr: float, beta1=0.9, beta2=0.999, eps=1e-8, exact: bool = False):
+def mlp_adam(P, M, V, G, t: int, lr: float, beta1=0.9, beta2=0.999, eps=1e-8):
     for a in (P, M, V, G):
         _check_dev(a, (MLP_NPARAM,))
-    rc = gpu_lib(exact).howmhd_mlp_adam(P.data_ptr(), M.data_ptr(), V.data_ptr(), G.data_ptr(), t, lr, beta1, beta2,
+    rc = gpu_lib().howmhd_mlp_adam(P.data_ptr(), M.data_ptr(), V.data_ptr(), G.data_ptr(), t, lr, beta1, beta2,
                                         eps, _stream())
     if rc:
         raise HowmhdError(f"howmhd_mlp_adam failed ({rc})")
 
 
-def build_kind(exact: bool = False) -> str:
-    return gpu_lib(exact).howmhd_gpu_build_kind().decode()
+def build_kind() -> str:
+    return gpu_lib().howmhd_gpu_build_kind().decode()
 
 
 def active(a: np.ndarray) -> np.ndarray:

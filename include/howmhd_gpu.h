@@ -42,9 +42,14 @@ enum {
 enum { HOWMHD_BC_PERIODIC = 0, HOWMHD_BC_OPEN = 1 };
 
 enum {
-    HOWMHD_PREC_FP64 = 0,  /* plain method, all arithmetic in FP64                */
-    HOWMHD_PREC_FP32 = 1   /* sweeps and CT EMFs in FP32; stage sums, E->p, floors
-                              and the face-B accumulation stay in FP64             */
+    HOWMHD_PREC_FP64 = 0,      /* plain method in FP64 with the reference's rounding:
+                                  no contraction to FMA, true divisions, the
+                                  reference's evaluation order (bit-for-bit equal
+                                  to a reference CPU run built without FMA)       */
+    HOWMHD_PREC_FP32 = 1,      /* sweeps and CT EMFs in FP32 (with FMA); stage sums,
+                                  E->p, floors and the face-B accumulation in FP64 */
+    HOWMHD_PREC_FP64_FAST = 2  /* FP64 with FMA contraction and reciprocal forms:
+                                  same method, different rounding               */
 };
 
 enum {

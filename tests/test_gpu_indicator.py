@@ -6,8 +6,8 @@
   * sample collection: features and labels of every interface against the
     restatement's primitives and WENO weights (periodic and open axes);
   * routing: every interface flagged == the plain method bit for bit, none
-    flagged == the restatement's linear route bit for bit (FP-contraction-off
-    build); a mixed routing keeps mass, momentum, energy and the CT face
+    flagged == the restatement's linear route bit for bit (plain FP64 form,
+    reference rounding); a mixed routing keeps mass, momentum, energy and the CT face
     divergence to round-off, and FP32 sweeps run routed.
 """
 import numpy as np
@@ -133,22 +133,16 @@ def test_collect_samples_match_restatement(problem, kw, steps, mixed, require_gp
             assert y.sum() < n
 
 
-def _pair(problem, kw, exact):
-    run, q0, p = _setup(problem, **kw)
-    gp = H.GpuSolver(p, exact=exact)
-    return run, q0, p, gp
-
-
 @pytest.mark.parametrize("problem,kw", [
     ("orszag_tang", dict(nx=32, ny=32, nz=1, cour=0.8)),
     ("mhd_blast3d", dict(nx=12, ny=14, nz=16, x1bc="open", cour=0.4)),
 ])
-def test_routing_extremes_exact_build(problem, kw, require_gpu):
+def test_routing_extremes_plain_fp64(problem, kw, require_gpu):
     run, q0, p = _setup(problem, **kw)
     print(f"[routing extremes] {problem} {kw}", flush=True)
     ref_w, ref_l = O.RefSolver(p), O.RefSolver(p)
     ref_l.set_route(1)
-    g_all, g_none = H.GpuSolver(p, exact=True), H.GpuSolver(p, exact=True)
+    g_all, g_none = H.GpuSolver(p), H.GpuSolver(p)
     g_all.set_indicator(_params(0, b3=50.0), 0.5, enable=2)
     g_none.set_indicator(_params(0, b3=-50.0), 0.5, enable=2)
     for s in (ref_w, ref_l, g_all, g_none):
