@@ -72,8 +72,9 @@ def parse_args():
 
 def traffic_from_profile(kernel_key: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
-    summary (profiles/traffic.txt: '<kernel> <bytes_per_launch>' lines), or None."""
-    f = ROOT / "profiles" / "traffic.txt"
+    summary of the benchmark workload (profiles/traffic_mhd_blast3d_256_fp64.txt:
+    '<kernel> <bytes_per_launch>' lines), or None."""
+    f = ROOT / "profiles" / "traffic_mhd_blast3d_256_fp64.txt"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():

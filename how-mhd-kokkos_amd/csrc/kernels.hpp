@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
     using T = SweepTile<DIR>;
     __shared__ UF<R> luf[7][T::CELLS];
     __shared__ R lo[O_COUNT][T::CELLS];
-    __shared__ uint16_t mlp_act[MODE == SW_ROUTED ? 4 : 1][MODE == SW_ROUTED ? 64 * MLP_ROW : 1];
+    __shared__ uint16_t mlp_act[MODE == SW_ROUTED ? 4 : 1][MODE == SW_ROUTED ? 16 * MLP_ROW : 1];
     const Geom& g = A.g;
     const int nd = g.n[DIR];
     const int per = g.per[DIR];
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
         BFrag W1, W2;
         load_bfrag(A.mlp->W1t, W1);
         load_bfrag(A.mlp->W2t, W2);
-        const float zl = mlp_forward_wave(f, mlp_act[tid / 64], W1, W2, *A.mlp);
+        const float zl = mlp_forward_wave16(f, mlp_act[tid / 64], W1, W2, *A.mlp);
         flagged = valid && (zl > A.mlp->logit_threshold);
         if (A.stats) {
             const unsigned long long fb = __ballot(flagged), vb = __ballot(valid);
