@@ -34,11 +34,21 @@ struct howmhd_gpu_ctx {
     void* O2[3] = {};
     void* O[3] = {};
     unsigned long long* dmax = nullptr;
-    // learned indicator: device weights, routing mode (0 off, 1 on, 2 on and
-    // counted), counter slots
+    // learned indicator: device weights, routing mode (0 off, 1 on with the
+    // plain-WENO fallback, 2 on for every step), counter slots (64 x 4)
     howmhd::MlpDev* mlp = nullptr;
     int ind_mode = 0;
+    bool route_now = false;              // the running step routes
     unsigned long long* istats = nullptr;
+    // mode 1: counts of the last routed step travel to pinned host memory
+    // behind an event; a step whose wavefronts were mostly flagged sends the
+    // following steps to the plain sweep
+    unsigned long long* istats_host = nullptr;
+    hipEvent_t istats_ev = nullptr;
+    bool istats_pending = false;
+    int fallback_left = 0;
+    long long acc[4] = {0, 0, 0, 0};     // flagged, total, waves flagged, waves total
+    long long routed_steps = 0, fallback_steps = 0;
     bool faces_ready = false;
     bool timing = false;
     struct Timed {

@@ -44,6 +44,34 @@ namespace {
 
 std::string g_create_error = "no error";
 
+// Plain-WENO fallback of indicator mode 1: a routed step in which at least
+// this fraction of wavefronts ran the full WENO5 sends the next
+// kFallbackSteps steps to the plain sweep (no network evaluation), after
+// which one step routes again.
+constexpr double kFallbackWaveFraction = 0.9;
+constexpr int kFallbackSteps = 8;
+
+// Folds the counter copy of the last routed step into the running totals
+// once it has arrived (wait: block until it has) and takes the fallback
+// decision from it.
+int fold_indicator_stats(howmhd_gpu_ctx* c, bool wait) {
+    if (!c->istats_pending) return HOWMHD_OK;
+    if (wait) {
+        HCHECK(c, hipEventSynchronize(c->istats_ev));
+    } else {
+        const hipError_t q = hipEventQuery(c->istats_ev);
+        if (q == hipErrorNotReady) return HOWMHD_OK;
+        HCHECK(c, q);
+    }
+    long long s[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 64; ++i)
+        for (int k = 0; k < 4; ++k) s[k] += (long long)c->istats_host[4 * i + k];
+    for (int k = 0; k < 4; ++k) c->acc[k] += s[k];
+    c->istats_pending = false;
+    if (s[3] > 0 && (double)s[2] >= kFallbackWaveFraction * (double)s[3]) c->fallback_left = kFallbackSteps;
+    return HOWMHD_OK;
+}
+
 void drain_timing(howmhd_gpu_ctx* c) {
     for (auto& t : c->timed) {
         float ms = 0.f;
@@ -190,6 +218,9 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     ok = ok && alloc((void**)&c->dmax, 4 * sizeof(unsigned long long));
     ok = ok && alloc((void**)&c->mlp, sizeof(MlpDev));
     ok = ok && alloc((void**)&c->istats, 64 * 4 * sizeof(unsigned long long));
+    ok = ok && hipHostMalloc((void**)&c->istats_host, 64 * 4 * sizeof(unsigned long long), hipHostMallocDefault) ==
+                   hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&c->istats_ev, hipEventDisableTiming) == hipSuccess;
     if (!ok) return cleanup_fail(HOWMHD_ENOMEM, "howmhd_gpu_create: device allocation failed");
     *out = c;
     return HOWMHD_OK;
@@ -212,6 +243,8 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
     if (c->dmax) (void)hipFree(c->dmax);
     if (c->mlp) (void)hipFree(c->mlp);
     if (c->istats) (void)hipFree(c->istats);
+    if (c->istats_host) (void)hipHostFree(c->istats_host);
+    if (c->istats_ev) (void)hipEventDestroy(c->istats_ev);
     drain_timing(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -285,10 +318,33 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
     auto stage = (c->p.precision == HOWMHD_PREC_FP32)        ? &stage_fp32
                  : (c->p.precision == HOWMHD_PREC_FP64_FAST) ? &stage_fp64fast
                                                              : &stage_fp64;
+    c->route_now = (c->ind_mode == 2);
+    if (c->ind_mode == 2) ++c->routed_steps;
+    if (c->ind_mode == 1) {
+        const int rc = fold_indicator_stats(c, false);
+        if (rc) return rc;
+        if (c->fallback_left > 0) {
+            --c->fallback_left;
+            ++c->fallback_steps;
+        } else {
+            c->route_now = true;
+            ++c->routed_steps;
+        }
+    }
     for (int iw = 1; iw <= 5; ++iw) {
         const int rc = stage(c, iw, dt);
         if (rc) return rc;
     }
+    if (c->ind_mode == 1 && c->route_now && !c->istats_pending) {
+        // this step's counters to the host, then clear them for the next
+        HCHECK(c, hipMemcpyAsync(c->istats_host, c->istats, 64 * 4 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, c->stream));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, c->istats, 256);
+        HCHECK(c, hipGetLastError());
+        HCHECK(c, hipEventRecord(c->istats_ev, c->stream));
+        c->istats_pending = true;
+    }
+    c->route_now = false;
     // q(0) <- q(5), faces likewise: exchange the buffers of stages 0 and 5
     std::swap(c->qid[0], c->qid[1]);
     c->qid[5] = c->qid[1];
@@ -432,6 +488,11 @@ int howmhd_gpu_collect_samples(howmhd_gpu_ctx* c, int dir, float* X, float* y, i
 int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float threshold, int enable) {
     if (!c) return HOWMHD_EINVAL;
     if (enable < 0 || enable > 2) return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: enable is 0, 1 or 2");
+    if (enable == 0) {
+        HCHECK(c, hipStreamSynchronize(c->stream));
+        c->ind_mode = 0;
+        return HOWMHD_OK;
+    }
     if (enable && !params) return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: null params");
     if (enable && !(threshold > 0.f && threshold < 1.f))
         return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: threshold must lie in (0, 1)");
@@ -455,8 +516,12 @@ int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float thres
         h.b3 = params[MLP_OFF_B3];
         h.logit_threshold = std::log(threshold / (1.f - threshold));
         HCHECK(c, hipMemcpy(c->mlp, &h, sizeof(h), hipMemcpyHostToDevice));
-        HCHECK(c, hipMemset(c->istats, 0, 64 * 4 * sizeof(unsigned long long)));
     }
+    HCHECK(c, hipMemset(c->istats, 0, 64 * 4 * sizeof(unsigned long long)));
+    c->istats_pending = false;
+    c->fallback_left = 0;
+    for (int k = 0; k < 4; ++k) c->acc[k] = 0;
+    c->routed_steps = c->fallback_steps = 0;
     c->ind_mode = enable;
     return HOWMHD_OK;
 }
@@ -464,17 +529,27 @@ int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float thres
 int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* c, int64_t* flagged, int64_t* total, int64_t* waves_flagged,
                                int64_t* waves_total) {
     if (!c || !flagged || !total || !waves_flagged || !waves_total) return HOWMHD_EINVAL;
-    unsigned long long h[64 * 4];
     HCHECK(c, hipStreamSynchronize(c->stream));
+    const int rc = fold_indicator_stats(c, true);
+    if (rc) return rc;
+    unsigned long long h[64 * 4];
     HCHECK(c, hipMemcpy(h, c->istats, sizeof(h), hipMemcpyDeviceToHost));
     HCHECK(c, hipMemset(c->istats, 0, sizeof(h)));
-    int64_t s[4] = {0, 0, 0, 0};
+    long long s[4] = {c->acc[0], c->acc[1], c->acc[2], c->acc[3]};
     for (int i = 0; i < 64; ++i)
-        for (int k = 0; k < 4; ++k) s[k] += (int64_t)h[4 * i + k];
+        for (int k = 0; k < 4; ++k) s[k] += (long long)h[4 * i + k];
+    for (int k = 0; k < 4; ++k) c->acc[k] = 0;
     *flagged = s[0];
     *total = s[1];
     *waves_flagged = s[2];
     *waves_total = s[3];
+    return HOWMHD_OK;
+}
+
+int howmhd_gpu_indicator_routing(howmhd_gpu_ctx* c, int64_t* routed_steps, int64_t* fallback_steps) {
+    if (!c || !routed_steps || !fallback_steps) return HOWMHD_EINVAL;
+    *routed_steps = c->routed_steps;
+    *fallback_steps = c->fallback_steps;
     return HOWMHD_OK;
 }
 

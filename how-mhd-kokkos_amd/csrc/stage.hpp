@@ -60,7 +60,7 @@ SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     a.g = c->g;
     a.ph = c->ph;
     a.mlp = c->mlp;
-    a.stats = (c->ind_mode == 2) ? c->istats : nullptr;
+    a.stats = c->route_now ? c->istats : nullptr;
     return a;
 }
 
@@ -70,7 +70,7 @@ int launch_sweeps(howmhd_gpu_ctx* c, const double* qsrc) {
         if (c->g.n[d] <= 1) continue;
         const SweepArgs<R> a = sweep_args<R>(c, d, qsrc);
         TimedLaunch tl(c, HOWMHD_K_SWEEP_X + d);
-        if (c->ind_mode) launch_sweep<R, SW_ROUTED>(c, d, a);
+        if (c->route_now) launch_sweep<R, SW_ROUTED>(c, d, a);
         else launch_sweep<R, SW_PLAIN>(c, d, a);
         HCHECK(c, hipGetLastError());
     }

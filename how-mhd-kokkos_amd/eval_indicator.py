@@ -9,7 +9,9 @@ cells).  Forms:
 
   fp32                 FP32 sweeps/CT, WENO at every interface
   fp32+indicator       the trained network routes unflagged interfaces to
-                       linear upwind-5 (the product form)
+                       linear upwind-5 (the product form: with the
+                       plain-WENO fallback for steps whose wavefronts are
+                       nearly all flagged)
   fp32+linear          every interface on linear upwind-5 (no indicator)
   fp32+untrained       a randomly initialised network whose threshold gives
                        the trained network's flagged fraction (ablation)
@@ -151,8 +153,11 @@ def run_case(name, time_steps=10, forms=None):
     base_speed = None
     for form in forms:
         prec, Pf, thf = spec[form]
+        # the product forms route with the plain-WENO fallback; the ablation
+        # forms route every step
+        mode = 1 if form in ("fp32+indicator", "fp64+indicator") else 2
         # speed from the initial state
-        g = _solver(run, prec, Pf, thf, enable=1)
+        g = _solver(run, prec, Pf, thf, enable=mode)
         g.upload_q0(q0)
         g.init_faces()
         g.ssprk(g.tstep())
@@ -166,9 +171,9 @@ def run_case(name, time_steps=10, forms=None):
         speed = zones * time_steps / el
         if form == "fp64":
             base_speed = speed
-        # accuracy along the judge's dt sequence
+        # accuracy along the judge's dt sequence (same routing mode)
         if Pf is not None:
-            g.set_indicator(Pf, thf, enable=2)
+            g.set_indicator(Pf, thf, enable=mode)
         g.upload_q0(q0)
         g.init_faces()
         ok = True
@@ -187,6 +192,8 @@ def run_case(name, time_steps=10, forms=None):
             st = g.indicator_stats()
             ent["flagged_fraction"] = round(st["flagged_fraction"], 4)
             ent["wavefront_flagged_fraction"] = round(st["wave_fraction"], 4)
+            ent["routed_steps"] = st["routed_steps"]
+            ent["fallback_steps"] = st["fallback_steps"]
         g.close()
         out["forms"][form] = ent
         log(f"[{name}] {form}: {json.dumps(ent)}")

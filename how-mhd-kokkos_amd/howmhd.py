@@ -122,6 +122,7 @@ GPU_FUNCS = {
                                              C.POINTER(C.c_int64)]),
     "howmhd_gpu_set_indicator": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_int]),
     "howmhd_gpu_indicator_stats": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int64)] * 4),
+    "howmhd_gpu_indicator_routing": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int64)] * 2),
 }
 
 MLP_NPARAM = 1697
@@ -304,7 +305,8 @@ class GpuSolver:
         return cnt.value
 
     def set_indicator(self, params, threshold: float = 0.5, enable: int = 1):
-        """Routing on (1), on and counted (2) or off (0) with host weights."""
+        """Routing with the plain-WENO fallback (1), routing on every step (2)
+        or off (0), with host weights."""
         if params is None:
             self._check(self.lib.howmhd_gpu_set_indicator(self.h, None, threshold, enable), "set_indicator")
             return
@@ -317,8 +319,11 @@ class GpuSolver:
         v = [C.c_int64() for _ in range(4)]
         self._check(self.lib.howmhd_gpu_indicator_stats(self.h, *[C.byref(x) for x in v]), "indicator_stats")
         f, t, wf, wt = (x.value for x in v)
+        r = [C.c_int64() for _ in range(2)]
+        self._check(self.lib.howmhd_gpu_indicator_routing(self.h, *[C.byref(x) for x in r]), "indicator_routing")
         return {"flagged": f, "total": t, "waves_flagged": wf, "waves_total": wt,
-                "flagged_fraction": f / t if t else 0.0, "wave_fraction": wf / wt if wt else 0.0}
+                "flagged_fraction": f / t if t else 0.0, "wave_fraction": wf / wt if wt else 0.0,
+                "routed_steps": r[0].value, "fallback_steps": r[1].value}
 
     def close(self):
         if getattr(self, "h", None):

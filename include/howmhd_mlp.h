@@ -52,19 +52,29 @@ HOWMHD_API int howmhd_gpu_collect_samples(howmhd_gpu_ctx* ctx, int dir, float* X
                                int64_t* count);
 
 /* Loads indicator weights (params on the host, HOWMHD_MLP_NPARAM floats) and
- * the flag threshold on the output probability (0 < threshold < 1); enable 1
- * turns routing on for the following steps, 2 also counts flagged interfaces
- * (howmhd_gpu_indicator_stats), 0 turns it off (params may be NULL then).
+ * the flag threshold on the output probability (0 < threshold < 1).
+ *   enable 1: routing with the plain-WENO fallback: each routed step counts
+ *             its wavefronts; when at least 90% of them ran the full WENO5
+ *             (the network found little to skip), the next 8 steps run the
+ *             plain sweep, then one step routes again.  The counts of a step
+ *             are read back asynchronously and take effect a step later.
+ *   enable 2: routing on every step (no fallback), counted.
+ *   enable 0: off (params may be NULL).
  * A context created with indicator HOWMHD_INDICATOR_MLP refuses to step until
- * weights are loaded.  Synchronises. */
+ * weights are loaded.  Resets the counters.  Synchronises. */
 HOWMHD_API int howmhd_gpu_set_indicator(howmhd_gpu_ctx* ctx, const float* params, float threshold, int enable);
 
 /* Interfaces the indicator flagged / interfaces evaluated since the last
  * call (or set_indicator), and the same count at wavefront granularity (a
  * wavefront runs the full WENO5 when any of its interfaces is flagged).
- * Counted only with enable == 2.  Synchronises and resets the counters. */
+ * Steps that ran the plain sweep (fallback) are not counted.  Synchronises
+ * and resets the counters. */
 HOWMHD_API int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* ctx, int64_t* flagged, int64_t* total,
                                int64_t* waves_flagged, int64_t* waves_total);
+
+/* Steps that routed and steps that ran the plain sweep as the fallback since
+ * set_indicator (enable 1; enable 2 counts every step as routed). */
+HOWMHD_API int howmhd_gpu_indicator_routing(howmhd_gpu_ctx* ctx, int64_t* routed_steps, int64_t* fallback_steps);
 
 #ifdef __cplusplus
 }

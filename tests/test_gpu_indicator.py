@@ -248,3 +248,42 @@ def test_trained_indicator_ablation(require_gpu):
     assert tr["flagged_fraction"] < 0.5
     assert (not lin["finite"]) or lin["rel_l1_rho"] >= 10 * tr["rel_l1_rho"]
     assert (not un["finite"]) or un["rel_l1_rho"] >= 10 * tr["rel_l1_rho"]
+
+
+def test_plain_weno_fallback(require_gpu):
+    """Routing mode 1: where nearly every wavefront is flagged (decaying
+    turbulence) the steps fall back to the plain sweep, re-checking every 9th
+    step; where few are flagged (blast) every step routes.  Steps in fallback
+    equal the plain method."""
+    import eval_indicator as E
+    P, thr = H.load_indicator(E.WEIGHTS)
+    # turbulence: nearly everything flagged
+    run, q0, p = _setup("turbulence3d", nx=32, ny=32, nz=32, cour=0.4, seed=2)
+    p.precision = H.PREC["fp32"]
+    g, plain = H.GpuSolver(p), H.GpuSolver(p)
+    g.set_indicator(P, thr, enable=1)
+    for s in (g, plain):
+        s.upload_q0(q0)
+        s.init_faces()
+    for _ in range(20):
+        dt = plain.tstep()
+        g.tstep()          # synchronises: the step counts arrive before the next step
+        plain.ssprk(dt)
+        g.ssprk(dt)
+    st = g.indicator_stats()
+    print("turbulence routing:", st)
+    assert st["fallback_steps"] >= 8 and st["routed_steps"] >= 2
+    assert st["routed_steps"] + st["fallback_steps"] == 20
+    assert H.rel_l1(g.q0(), plain.q0())[0] <= 1e-4
+    # blast: few interfaces flagged, no fallback
+    run, q0, p = _setup("mhd_blast3d", nx=32, ny=32, nz=32, cour=0.4)
+    p.precision = H.PREC["fp32"]
+    g = H.GpuSolver(p)
+    g.set_indicator(P, thr, enable=1)
+    g.upload_q0(q0)
+    g.init_faces()
+    for _ in range(6):
+        g.ssprk(g.tstep())
+    st = g.indicator_stats()
+    print("blast routing:", st)
+    assert st["fallback_steps"] == 0 and st["routed_steps"] == 6
