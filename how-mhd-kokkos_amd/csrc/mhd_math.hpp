@@ -38,14 +38,26 @@ __device__ __forceinline__ R rabs(R x) { return fabs(x); }
 template <>
 __device__ __forceinline__ float rabs<float>(float x) { return fabsf(x); }
 
-// x / c for a compile-time constant c.  Exact build: a true division, as the
-// reference writes it.  Product build: multiplication by the rounded
-// reciprocal (differs by at most one rounding).
+// a / b from y = RN(1/b), correctly rounded: q = a*y is within one ulp of
+// a/b, the FMA remainder r = a - b q is exact, and RN(q + r y) = RN(a/b)
+// (Markstein's theorem; barring overflow and underflow).  When r == 0 the
+// quotient q is exact (and keeps the sign of a zero numerator).  So the
+// result is the IEEE quotient bit for bit, in 6 instructions instead of an
+// 11-instruction division when y is shared or constant.
+template <typename R>
+__device__ __forceinline__ R div_by(R a, R b, R y) {
+    const R q = a * y;
+    const R r = __builtin_fma(-b, q, a);
+    return (r == R(0)) ? q : __builtin_fma(r, y, q);
+}
+
+// x / c for a compile-time constant c with inv_c = RN(1/c).  Reference
+// rounding: the correctly rounded quotient (div_by).  FMA form:
+// multiplication by the rounded reciprocal (differs by at most one rounding).
 template <typename R>
 __device__ __forceinline__ R div_c(R x, R c, R inv_c) {
 #if HOWMHD_EXACT
-    (void)inv_c;
-    return x / c;
+    return div_by<R>(x, c, inv_c);
 #else
     (void)c;
     return x * inv_c;
@@ -81,9 +93,11 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     double EE = qs[7];
     double rho = DD;
 #if HOWMHD_EXACT
-    const double vx = Mx / rho;
-    const double vy = My / rho;
-    const double vz = Mz / rho;
+    // correctly rounded quotients from shared reciprocals (div_by)
+    const double iDD = 1.0 / DD;
+    const double vx = div_by(Mx, DD, iDD);
+    const double vy = div_by(My, DD, iDD);
+    const double vz = div_by(Mz, DD, iDD);
 #else
     double irho = 1.0 / rho;
     const double vx = Mx * irho;
@@ -104,7 +118,8 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
 #endif
     }
 #if HOWMHD_EXACT
-    const double HH = (EE + pg) / rho;
+    const double irho = (rho == DD) ? iDD : 1.0 / rho;  // floored rho
+    const double HH = div_by(EE + pg, rho, irho);
 #else
     const double HH = (EE + pg) * irho;
 #endif
@@ -118,9 +133,10 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const R pt = rpg + R(0.5) * rBB2;
     const R vdB = rvx * rBx + rvy * rBy + rvz * rBz;
 #if HOWMHD_EXACT
-    const R bbn2 = rBB2 / rrho;
-    const R bnx2 = rBx * rBx / rrho;
-    const R cs2 = rmax(R(0), rgam * rabs(rpg / rrho));
+    static_assert(sizeof(R) == 8, "the reference-rounding form is FP64");
+    const R bbn2 = div_by<R>(rBB2, rrho, irho);
+    const R bnx2 = div_by<R>(rBx * rBx, rrho, irho);
+    const R cs2 = rmax(R(0), rgam * rabs(div_by<R>(rpg, rrho, irho)));
 #else
     const R rirho = R(irho);
     const R bbn2 = rBB2 * rirho;
@@ -152,7 +168,7 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
 template <typename R>
 struct Eig {
     R vx, vy, vz, By, Bz, vv2, cs, cs2, lf, ls, af, as, bty, btz, sgnBx, sqrtrho, invsqrtrho, sgnBt;
-    R nrm_half, nrm_ent;  // product build: 0.5/cs2 and (gam-1)/cs2, the row normalisers
+    R nrm_half, nrm_ent;  // FMA form: 0.5/cs2 and (gam-1)/cs2; reference rounding: nrm_half = RN(1/cs2)
     bool slow_flip;  // cs >= la: waves 2 and 4 take the sign of Bt; else waves 0 and 6
 };
 
@@ -225,7 +241,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     e.nrm_half = R(0.5) * ics2;
     e.nrm_ent = (gam - R(1)) * ics2;
 #else
-    e.nrm_half = R(0);
+    e.nrm_half = R(1) / cs2;   // RN(1/cs2): the L-row normalisation divides by cs2 through div_by
     e.nrm_ent = R(0);
 #endif
 }
@@ -301,8 +317,8 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
     for (int c = 0; c < 7; ++c) {
 #if HOWMHD_EXACT
         if constexpr (M == 1 || M == 5) L[c] = R(0.5) * L[c];
-        else if constexpr (M == 3) L[c] = -gam0 * L[c] / cs2;
-        else L[c] = R(0.5) * L[c] / cs2;
+        else if constexpr (M == 3) L[c] = div_by<R>(-gam0 * L[c], cs2, e.nrm_half);
+        else L[c] = div_by<R>(R(0.5) * L[c], cs2, e.nrm_half);
 #else
         if constexpr (M == 1 || M == 5) L[c] = R(0.5) * L[c];
         else if constexpr (M == 3) L[c] = L[c] * e.nrm_ent;
@@ -418,7 +434,8 @@ __device__ __forceinline__ R weno_side(R a, R b, R c, R d, R* w0_out = nullptr, 
     const R omega0 = alpha0 / (alpha0 + alpha1 + alpha2);
     const R omega2 = alpha2 / (alpha0 + alpha1 + alpha2);
     if (w0_out) { *w0_out = omega0; *w2_out = omega2; }
-    return omega0 * (a - R(2) * b + c) / R(3) + (omega2 - R(0.5)) * (b - R(2) * c + d) / R(6);
+    return div_c<R>(omega0 * (a - R(2) * b + c), R(3), R(1.0 / 3.0)) +
+           div_c<R>((omega2 - R(0.5)) * (b - R(2) * c + d), R(6), R(1.0 / 6.0));
 #else
     // alpha_k = d_k / t_k with t_k = (eps+IS_k)^2: the normalised weights are
     // w0 = t1 t2 / S, w2 = 3 t0 t1 / S with S = t1 t2 + 6 t0 t2 + 3 t0 t1,
