@@ -370,7 +370,17 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
     }
     const R gam = R(A.ph.gam);
     Eig<R> e;
-    eig_scalars<R>(gam, u0, u1, e);
+    if constexpr (sizeof(R) == 4) {
+        // FP32 form: interface scalars in FP64 (eig_round)
+        double d0[9], d1[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { d0[k] = (double)u0[k]; d1[k] = (double)u1[k]; }
+        Eig<double> ed;
+        eig_scalars<double>(A.ph.gam, d0, d1, ed);
+        eig_round<R>(ed, e);
+    } else {
+        eig_scalars<R>(gam, u0, u1, e);
+    }
 
     // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
     Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};

@@ -137,16 +137,22 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const R bbn2 = div_by<R>(rBB2, rrho, irho);
     const R bnx2 = div_by<R>(rBx * rBx, rrho, irho);
     const R cs2 = rmax(R(0), rgam * rabs(div_by<R>(rpg, rrho, irho)));
-#else
-    const R rirho = R(irho);
-    const R bbn2 = rBB2 * rirho;
-    const R bnx2 = rBx * rBx * rirho;
-    const R cs2 = rmax(R(0), rgam * rabs(rpg * rirho));
-#endif
     const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
     const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
     const R la = rsqrt_(rmax(R(0), bnx2));
     const R ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
+#else
+    // wave speeds in FP64 for every form: (b^2 + cs^2)^2 - 4 bn^2 cs^2 cancels
+    // where the fast and slow speeds meet
+    (void)rgam;
+    const double bbn2 = BB2 * irho;
+    const double bnx2 = Bx * Bx * irho;
+    const double cs2 = fmax(0.0, gam * fabs(pg * irho));
+    const double root = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+    const R lf = R(sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + root))));
+    const R la = R(sqrt(fmax(0.0, bnx2)));
+    const R ls = R(sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 - root))));
+#endif
 
     out[C_RHO] = rDD;  out[C_MN] = rMx;  out[C_MT1] = rMy;  out[C_MT2] = rMz;
     out[C_BT1] = rBy;  out[C_BT2] = rBz; out[C_E] = rEE;
@@ -244,6 +250,19 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     e.nrm_half = R(1) / cs2;   // RN(1/cs2): the L-row normalisation divides by cs2 through div_by
     e.nrm_ent = R(0);
 #endif
+}
+
+// Eig<double> rounded to R (the FP32 form builds the interface scalars in
+// FP64: the speeds and the alpha_f/alpha_s and beta_t normalisations cancel
+// near degenerate points, where FP32 rounding gives eigenvectors with L R far
+// from I).
+template <typename R>
+__device__ __forceinline__ void eig_round(const Eig<double>& d, Eig<R>& e) {
+    e.vx = R(d.vx); e.vy = R(d.vy); e.vz = R(d.vz); e.By = R(d.By); e.Bz = R(d.Bz); e.vv2 = R(d.vv2);
+    e.cs = R(d.cs); e.cs2 = R(d.cs2); e.lf = R(d.lf); e.ls = R(d.ls); e.af = R(d.af); e.as = R(d.as);
+    e.bty = R(d.bty); e.btz = R(d.btz); e.sgnBx = R(d.sgnBx); e.sqrtrho = R(d.sqrtrho);
+    e.invsqrtrho = R(d.invsqrtrho); e.sgnBt = R(d.sgnBt); e.nrm_half = R(d.nrm_half); e.nrm_ent = R(d.nrm_ent);
+    e.slow_flip = d.slow_flip;
 }
 
 // Row m of L after normalisation and sign flip (src/ssprk.cpp:662-726,
