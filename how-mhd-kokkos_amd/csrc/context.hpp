@@ -100,8 +100,9 @@ struct TimedLaunch {
 int stage_fp64(howmhd_gpu_ctx* c, int iw, double dt);      // step_exact.hip
 int stage_fp64fast(howmhd_gpu_ctx* c, int iw, double dt);  // step_fast.hip
 int stage_fp32(howmhd_gpu_ctx* c, int iw, double dt);      // step_fast.hip
-// Indicator samples of the stage-0 state along axis dir from the plain FP64
-// sweep (step_exact.hip).
-int collect_fp64(howmhd_gpu_ctx* c, int dir, float* X, float* y);
+// Indicator samples (X, y) and/or the in-sweep network's logits of the
+// stage-0 state along axis dir from the plain FP64 sweep (step_exact.hip);
+// null outputs are skipped.
+int collect_fp64(howmhd_gpu_ctx* c, int dir, float* X, float* y, float* logits);
 
 }  // namespace howmhd

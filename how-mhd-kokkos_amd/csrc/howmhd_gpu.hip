@@ -478,7 +478,26 @@ int howmhd_gpu_collect_samples(howmhd_gpu_ctx* c, int dir, float* X, float* y, i
         return fail(c, HOWMHD_EINVAL, "howmhd_gpu_collect_samples: labels come from the plain FP64 solver (precision fp64)");
     // the sweep writes this direction's dF and CT terms too; the next stage
     // recomputes them from its own source
-    const int rc = collect_fp64(c, dir, X, y);
+    const int rc = collect_fp64(c, dir, X, y, nullptr);
+    if (rc) return rc;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    *count = need;
+    return HOWMHD_OK;
+}
+
+int howmhd_gpu_indicator_logits(howmhd_gpu_ctx* c, int dir, float* logits, int64_t capacity, int64_t* count) {
+    if (!c || !logits || !count || dir < 0 || dir > 2) return HOWMHD_EINVAL;
+    *count = 0;
+    if (!c->ind_mode) return fail(c, HOWMHD_ESTATE, "howmhd_gpu_indicator_logits: no weights loaded");
+    const int n = c->g.n[dir];
+    if (n <= 1) return HOWMHD_OK;
+    const int64_t need = (int64_t)(c->ncell / n) * (c->g.per[dir] ? n : n + 1);
+    if (capacity < need)
+        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_indicator_logits: capacity " + std::to_string(capacity) + " < " +
+                                          std::to_string(need) + " interfaces");
+    if (c->p.precision != HOWMHD_PREC_FP64)
+        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_indicator_logits: needs precision fp64");
+    const int rc = collect_fp64(c, dir, nullptr, nullptr, logits);
     if (rc) return rc;
     HCHECK(c, hipStreamSynchronize(c->stream));
     *count = need;

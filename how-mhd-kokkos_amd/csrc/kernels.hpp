@@ -107,9 +107,11 @@ struct SweepArgs {
     R* ct_b;            // CT term "bsz": fsz (x), gsx (y), hsy (z)
     Geom g;
     Phys ph;
-    // SW_COLLECT: 18 features and the label per interface (sample_index)
+    // SW_COLLECT: 18 features and the label per interface (sample_index), and
+    // (when logit is set) the in-sweep network's logit
     float* feat;
     float* label;
+    float* logit;
     // SW_ROUTED: device weights; stats (null: not counted) holds 64 slots of
     // (flagged, total, wavefronts flagged, wavefronts total), summed by the host
     const MlpDev* mlp;
@@ -249,7 +251,9 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
     using T = SweepTile<DIR>;
     __shared__ UF<R> luf[7][T::CELLS];
     __shared__ R lo[O_COUNT][T::CELLS];
-    __shared__ uint16_t mlp_act[MODE == SW_ROUTED ? 4 : 1][MODE == SW_ROUTED ? 16 * MLP_ROW : 1];
+    __shared__ __attribute__((aligned(16))) uint16_t mlp_act[MODE != SW_PLAIN ? 4 : 1][MODE != SW_PLAIN ? 16 * MLP_ROWT : 1];
+    // |B| of each cell for the indicator features (routed and sample sweeps)
+    __shared__ float lbmag[MODE != SW_PLAIN ? T::CELLS : 1];
     const Geom& g = A.g;
     const int nd = g.n[DIR];
     const int per = g.per[DIR];
@@ -300,6 +304,8 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
         for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
 #pragma unroll
         for (int f = 0; f < O_COUNT; ++f) lo[f][s] = out[C_VN + f];
+        if constexpr (MODE != SW_PLAIN)
+            lbmag[s] = sqrtf((float)(out[C_BN] * out[C_BN] + out[C_BT1] * out[C_BT1] + out[C_BT2] * out[C_BT2]));
     }
     __syncthreads();
 
@@ -327,21 +333,31 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const int s = valid ? slot(k) : 0;
-            const R bn = lo[O_BN][s], b1 = luf[4][s].u, b2 = luf[5][s].u;
             rr[k] = (float)luf[0][s].u;
             pp[k] = (float)lo[O_P][s];
-            bb[k] = sqrtf((float)(bn * bn + b1 * b1 + b2 * b2));
+            bb[k] = lbmag[s];
         }
         mlp_features(rr, pp, bb, f);
     };
     bool flagged = true;
+    if constexpr (MODE == SW_COLLECT) {
+        if (A.logit) {   // uniform over the launch
+            float f[MLP_IN];
+            features(f);
+            BFrag W1, W2;
+            load_bfrag(A.mlp->W1t, W1);
+            load_bfrag(A.mlp->W2t, W2);
+            const float zl = mlp_forward_wave_t(f, mlp_act[tid / 64], W1, W2, *A.mlp);
+            if (valid) A.logit[sample_index<DIR>(g, t, x, y, z)] = zl;
+        }
+    }
     if constexpr (MODE == SW_ROUTED) {
         float f[MLP_IN];
         features(f);
         BFrag W1, W2;
         load_bfrag(A.mlp->W1t, W1);
         load_bfrag(A.mlp->W2t, W2);
-        const float zl = mlp_forward_wave16(f, mlp_act[tid / 64], W1, W2, *A.mlp);
+        const float zl = mlp_forward_wave_t(f, mlp_act[tid / 64], W1, W2, *A.mlp);
         flagged = valid && (zl > A.mlp->logit_threshold);
         if (A.stats) {
             const unsigned long long fb = __ballot(flagged), vb = __ballot(valid);
@@ -370,17 +386,7 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
     }
     const R gam = R(A.ph.gam);
     Eig<R> e;
-    if constexpr (sizeof(R) == 4) {
-        // FP32 form: interface scalars in FP64 (eig_round)
-        double d0[9], d1[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) { d0[k] = (double)u0[k]; d1[k] = (double)u1[k]; }
-        Eig<double> ed;
-        eig_scalars<double>(A.ph.gam, d0, d1, ed);
-        eig_round<R>(ed, e);
-    } else {
-        eig_scalars<R>(gam, u0, u1, e);
-    }
+    eig_scalars<R>(gam, u0, u1, e);
 
     // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
     Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
@@ -501,6 +507,7 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
         A.ct_b[ci] = dF[5] + ftmp2;
     }
     if constexpr (MODE == SW_COLLECT) {
+        if (!A.feat) return;
         float f[MLP_IN];
         features(f);
         const long long si = sample_index<DIR>(g, t, x, y, z);

@@ -142,16 +142,31 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const R la = rsqrt_(rmax(R(0), bnx2));
     const R ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
 #else
-    // wave speeds in FP64 for every form: (b^2 + cs^2)^2 - 4 bn^2 cs^2 cancels
-    // where the fast and slow speeds meet
-    (void)rgam;
-    const double bbn2 = BB2 * irho;
-    const double bnx2 = Bx * Bx * irho;
-    const double cs2 = fmax(0.0, gam * fabs(pg * irho));
-    const double root = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
-    const R lf = R(sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + root))));
-    const R la = R(sqrt(fmax(0.0, bnx2)));
-    const R ls = R(sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 - root))));
+    R lf, la, ls;
+    if constexpr (sizeof(R) == 8) {
+        const R rirho = R(irho);
+        const R bbn2 = rBB2 * rirho;
+        const R bnx2 = rBx * rBx * rirho;
+        const R cs2 = rmax(R(0), rgam * rabs(rpg * rirho));
+        const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
+        lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
+        la = rsqrt_(rmax(R(0), bnx2));
+        ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
+    } else {
+        // FP32: the same speeds from cancellation-free forms,
+        // root^2 = (b^2 - cs^2)^2 + 4 bt^2 cs^2 and cs_slow^2 = bn^2 cs^2 / cf^2
+        const R rirho = R(irho);
+        const R bnx2 = rBx * rBx * rirho;
+        const R btr2 = (rBy * rBy + rBz * rBz) * rirho;
+        const R bbn2 = bnx2 + btr2;
+        const R cs2 = rmax(R(0), rgam * rabs(rpg * rirho));
+        const R dd = bbn2 - cs2;
+        const R root = rsqrt_(dd * dd + R(4) * btr2 * cs2);
+        const R lf2 = R(0.5) * (bbn2 + cs2 + root);
+        lf = rsqrt_(lf2);
+        la = rsqrt_(bnx2);
+        ls = (lf2 > R(0)) ? rsqrt_(bnx2 * cs2 / lf2) : R(0);
+    }
 #endif
 
     out[C_RHO] = rDD;  out[C_MN] = rMx;  out[C_MT1] = rMy;  out[C_MT2] = rMz;
@@ -178,6 +193,36 @@ struct Eig {
     bool slow_flip;  // cs >= la: waves 2 and 4 take the sign of Bt; else waves 0 and 6
 };
 
+// FP32 form: fast/slow speeds and the alpha_f, alpha_s normalisations of
+// src/ssprk.cpp:627-656 from cancellation-free forms of the same quantities
+// (b^2 = bn^2 + bt^2 per unit density):
+//   root^2 = (b^2 - cs^2)^2 + 4 bt^2 cs^2,  cf^2 = (b^2 + cs^2 + root)/2,
+//   cs_slow^2 = bn^2 cs^2 / cf^2,
+//   af^2 = (cs^2 - b^2 + root) / (2 root),  as^2 = (b^2 - cs^2 + root) / (2 root),
+// where a numerator that would cancel is taken as 4 bt^2 cs^2 / (root + |b^2 - cs^2|).
+// In FP32 the reference forms lose every digit near the degenerate points and
+// give eigenvectors with L R far from I.
+template <typename R>
+__device__ __forceinline__ void eig_scalars_fp32_speeds(Eig<R>& e, R cs2, R bn2, R bt2, R b2) {
+    const R c2 = rmax(R(0), cs2);
+    const R dd = b2 - c2;
+    const R root = rsqrt_(dd * dd + R(4) * bt2 * c2);
+    const R lf2 = R(0.5) * (b2 + c2 + root);
+    e.lf = rsqrt_(lf2);
+    e.ls = (lf2 > R(0)) ? rsqrt_(bn2 * c2 / lf2) : R(0);
+    if (root >= R(1.0e-30)) {
+        const R q = R(4) * bt2 * c2;
+        const R fnum = (dd <= R(0)) ? root - dd : q / (root + dd);   // cs^2 - b^2 + root
+        const R snum = (dd >= R(0)) ? root + dd : q / (root - dd);   // b^2 - cs^2 + root
+        const R inv = R(0.5) / root;
+        e.af = rsqrt_(rmax(R(0), fnum * inv));
+        e.as = rsqrt_(rmax(R(0), snum * inv));
+    } else {
+        e.af = R(1);
+        e.as = R(1);
+    }
+}
+
 template <typename R>
 __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig<R>& e) {
     // u = (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H) of the two cells
@@ -201,25 +246,17 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
 #endif
     const R cs2 = (gam - R(1)) * (HH - R(0.5) * (vv2 + bbn2));
     const R cs = rsqrt_(rmax(R(0), cs2));
+    const R Bt2 = By * By + Bz * Bz;
+#if !HOWMHD_EXACT
+    if constexpr (sizeof(R) == 4) {
+        eig_scalars_fp32_speeds(e, cs2, bnx2, Bt2 * irho, bbn2);
+    } else
+#endif
+    {
     const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
     const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
-    const R la = rsqrt_(rmax(R(0), bnx2));
     const R ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
-    const R Bt2 = By * By + Bz * Bz;
-    R bty, btz;
-    if (Bt2 >= R(1.0e-30)) {
-#if HOWMHD_EXACT
-        bty = By / rsqrt_(Bt2);
-        btz = Bz / rsqrt_(Bt2);
-#else
-        const R rb = R(1) / rsqrt_(Bt2);
-        bty = By * rb;
-        btz = Bz * rb;
-#endif
-    } else {
-        bty = R(1) / rsqrt_(R(2));
-        btz = R(1) / rsqrt_(R(2));
-    }
+    e.lf = lf; e.ls = ls;
     R af, as;
     if ((lf * lf - ls * ls) >= R(1.0e-30)) {
 #if HOWMHD_EXACT
@@ -234,9 +271,26 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
         af = R(1);
         as = R(1);
     }
+    e.af = af; e.as = as;
+    }
+    const R la = rsqrt_(rmax(R(0), bnx2));
+    R bty, btz;
+    if (Bt2 >= R(1.0e-30)) {
+#if HOWMHD_EXACT
+        bty = By / rsqrt_(Bt2);
+        btz = Bz / rsqrt_(Bt2);
+#else
+        const R rb = R(1) / rsqrt_(Bt2);
+        bty = By * rb;
+        btz = Bz * rb;
+#endif
+    } else {
+        bty = R(1) / rsqrt_(R(2));
+        btz = R(1) / rsqrt_(R(2));
+    }
     const R sqrtrho = rsqrt_(rho);
     e.vx = vx; e.vy = vy; e.vz = vz; e.By = By; e.Bz = Bz; e.vv2 = vv2;
-    e.cs = cs; e.cs2 = cs2; e.lf = lf; e.ls = ls; e.af = af; e.as = as;
+    e.cs = cs; e.cs2 = cs2;
     e.bty = bty; e.btz = btz; e.sgnBx = sign1(Bx);
     e.sqrtrho = sqrtrho;
     e.invsqrtrho = R(1) / sqrtrho;
@@ -250,19 +304,6 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     e.nrm_half = R(1) / cs2;   // RN(1/cs2): the L-row normalisation divides by cs2 through div_by
     e.nrm_ent = R(0);
 #endif
-}
-
-// Eig<double> rounded to R (the FP32 form builds the interface scalars in
-// FP64: the speeds and the alpha_f/alpha_s and beta_t normalisations cancel
-// near degenerate points, where FP32 rounding gives eigenvectors with L R far
-// from I).
-template <typename R>
-__device__ __forceinline__ void eig_round(const Eig<double>& d, Eig<R>& e) {
-    e.vx = R(d.vx); e.vy = R(d.vy); e.vz = R(d.vz); e.By = R(d.By); e.Bz = R(d.Bz); e.vv2 = R(d.vv2);
-    e.cs = R(d.cs); e.cs2 = R(d.cs2); e.lf = R(d.lf); e.ls = R(d.ls); e.af = R(d.af); e.as = R(d.as);
-    e.bty = R(d.bty); e.btz = R(d.btz); e.sgnBx = R(d.sgnBx); e.sqrtrho = R(d.sqrtrho);
-    e.invsqrtrho = R(d.invsqrtrho); e.sgnBt = R(d.sgnBt); e.nrm_half = R(d.nrm_half); e.nrm_ent = R(d.nrm_ent);
-    e.slow_flip = d.slow_flip;
 }
 
 // Row m of L after normalisation and sign flip (src/ssprk.cpp:662-726,

@@ -196,4 +196,78 @@ __device__ __forceinline__ float mlp_forward_wave16(const float* feat, uint16_t*
     return zl;
 }
 
+// The forward pass the routed sweep uses.  Transposed products: each layer is
+// D = W A^T, with the weights as the MFMA A operand (the B fragments of
+// load_bfrag hold exactly W[16u + l%16][16s + 4(l/16) + v]) and the
+// activations of 16 samples as the B operand, so that every lane reads and
+// writes its activations as 4 consecutive bf16 (64-bit LDS accesses); the
+// output layer is a third product whose A operand is w3 in row 0, and the
+// logits are moved to their lanes with a cross-lane permute.
+// act: 16 x MLP_ROWT bf16 per wavefront (8-byte aligned rows).
+constexpr int MLP_ROWT = 36;
+
+__device__ __forceinline__ bf16x4 lds_b64(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+__device__ __forceinline__ void layer_relu_t(uint16_t* act, const BFrag& W, const float* bias) {
+    const int l = threadIdx.x & 63;
+    const int r16 = l % 16, kq = 4 * (l / 16);
+    const bf16x4 b0 = lds_b64(act + r16 * MLP_ROWT + kq);
+    const bf16x4 b1 = lds_b64(act + r16 * MLP_ROWT + 16 + kq);
+    f32x4 d[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        f32x4 c = {bias[16 * u + kq], bias[16 * u + kq + 1], bias[16 * u + kq + 2], bias[16 * u + kq + 3]};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W.f[0][u], b0, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W.f[1][u], b1, c, 0, 0, 0);
+        d[u] = c;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        bf16x4 o;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = (short)f2bf(d[u][v] > 0.f ? d[u][v] : 0.f);
+        *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + 16 * u + kq) = o;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ float mlp_forward_wave_t(const float* feat, uint16_t* act, const BFrag& W1,
+                                                    const BFrag& W2, const MlpDev& m) {
+    const int l = threadIdx.x & 63;
+    const int grp = l >> 4, r16 = l & 15, kq = 4 * (l >> 4);
+    // w3 as the A operand: row 0 only
+    bf16x4 w3f[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w3f[s][v] = (short)(r16 == 0 ? f2bf(m.w3[16 * s + kq + v]) : 0);
+    float zl = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (grp == t) {
+#pragma unroll
+            for (int j = 0; j < MLP_H; j += 4) {
+                bf16x4 o;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) o[v] = (short)((j + v < MLP_IN) ? f2bf(feat[(j + v < MLP_IN) ? j + v : 0]) : 0);
+                *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + j) = o;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        layer_relu_t(act, W1, m.b1);
+        layer_relu_t(act, W2, m.b2);
+        const bf16x4 b0 = lds_b64(act + r16 * MLP_ROWT + kq);
+        const bf16x4 b1 = lds_b64(act + r16 * MLP_ROWT + 16 + kq);
+        f32x4 c = {kq == 0 ? m.b3 : 0.f, 0.f, 0.f, 0.f};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3f[0], b0, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3f[1], b1, c, 0, 0, 0);
+        // lanes 0..15 hold z3 of samples 0..15 of this tile in c[0]
+        const float z = __shfl(c[0], r16, 64);
+        if (grp == t) zl = z;
+        __builtin_amdgcn_wave_barrier();
+    }
+    return zl;
+}
+
 }  // namespace howmhd

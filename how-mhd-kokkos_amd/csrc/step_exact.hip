@@ -13,10 +13,11 @@ namespace howmhd {
 
 int stage_fp64(howmhd_gpu_ctx* c, int iw, double dt) { return launch_stage<double>(c, iw, dt); }
 
-int collect_fp64(howmhd_gpu_ctx* c, int dir, float* X, float* y) {
+int collect_fp64(howmhd_gpu_ctx* c, int dir, float* X, float* y, float* logits) {
     SweepArgs<double> a = sweep_args<double>(c, dir, c->qbuf[c->qid[0]]);
     a.feat = X;
     a.label = y;
+    a.logit = logits;
     a.stats = nullptr;
     launch_sweep<double, SW_COLLECT>(c, dir, a);
     HCHECK(c, hipGetLastError());

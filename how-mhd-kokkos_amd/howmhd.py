@@ -123,6 +123,7 @@ GPU_FUNCS = {
     "howmhd_gpu_set_indicator": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_int]),
     "howmhd_gpu_indicator_stats": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int64)] * 4),
     "howmhd_gpu_indicator_routing": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int64)] * 2),
+    "howmhd_gpu_indicator_logits": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
 }
 
 MLP_NPARAM = 1697
@@ -302,6 +303,16 @@ class GpuSolver:
         cnt = C.c_int64()
         self._check(self.lib.howmhd_gpu_collect_samples(self.h, d, X.data_ptr(), y.data_ptr(), X.shape[0],
                                                         C.byref(cnt)), "collect_samples")
+        return cnt.value
+
+    def indicator_logits(self, d: int, z) -> int:
+        """The in-sweep network's logits of every interface along axis d into
+        the float32 device tensor z (torch); returns the count."""
+        _check_dev(z, (None,))
+        _torch().cuda.synchronize()
+        cnt = C.c_int64()
+        self._check(self.lib.howmhd_gpu_indicator_logits(self.h, d, z.data_ptr(), z.shape[0], C.byref(cnt)),
+                    "indicator_logits")
         return cnt.value
 
     def set_indicator(self, params, threshold: float = 0.5, enable: int = 1):

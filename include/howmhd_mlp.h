@@ -51,6 +51,14 @@ HOWMHD_API int howmhd_mlp_adam(float* params, float* m, float* v, const float* g
 HOWMHD_API int howmhd_gpu_collect_samples(howmhd_gpu_ctx* ctx, int dir, float* X, float* y, int64_t capacity,
                                int64_t* count);
 
+/* The network's logit z3 for every interface of the sweep along `dir` of the
+ * current stage-0 state, as the routed sweep computes it (features from the
+ * sweep's LDS tile, MFMA forward pass), in the sample order of
+ * howmhd_gpu_collect_samples; logits [dev].  Needs weights loaded
+ * (set_indicator) and precision fp64.  Synchronises. */
+HOWMHD_API int howmhd_gpu_indicator_logits(howmhd_gpu_ctx* ctx, int dir, float* logits, int64_t capacity,
+                                           int64_t* count);
+
 /* Loads indicator weights (params on the host, HOWMHD_MLP_NPARAM floats) and
  * the flag threshold on the output probability (0 < threshold < 1).
  *   enable 1: routing with the plain-WENO fallback: each routed step counts

@@ -287,3 +287,34 @@ def test_plain_weno_fallback(require_gpu):
     st = g.indicator_stats()
     print("blast routing:", st)
     assert st["fallback_steps"] == 0 and st["routed_steps"] == 6
+
+
+def test_in_sweep_network_matches_standalone_kernel(require_gpu):
+    """The routed sweep's network (transposed MFMA products on the sweep's
+    LDS tile) gives the standalone inference kernel's logits on the same
+    features."""
+    torch = _torch()
+    import eval_indicator as E
+    P, thr = H.load_indicator(E.WEIGHTS)
+    run, q0, p = _setup("mhd_blast3d", nx=24, ny=20, nz=16, x1bc="open", cour=0.4)
+    g = H.GpuSolver(p)
+    g.set_indicator(P, thr, enable=2)
+    g.upload_q0(q0)
+    g.init_faces()
+    for _ in range(3):
+        g.ssprk(g.tstep())
+    Pd = torch.from_numpy(P).cuda()
+    for d in range(3):
+        n = g.n_samples(d)
+        X = torch.zeros((n, 18), dtype=torch.float32, device="cuda")
+        y = torch.zeros(n, dtype=torch.float32, device="cuda")
+        zs = torch.zeros(n, dtype=torch.float32, device="cuda")
+        assert g.collect_samples(d, X, y) == n
+        assert g.indicator_logits(d, zs) == n
+        zr = H.mlp_infer(X, Pd).cpu().numpy()
+        zs = zs.cpu().numpy()
+        dz = np.abs(zs - zr)
+        scale = 1 + np.abs(zr)
+        print(f"dir {d}: max |dz| {dz.max():.3e}, mean {dz.mean():.3e}")
+        assert np.all(dz <= 2e-2 * scale)
+        assert np.mean(dz <= 1e-4 * scale) >= 0.99
