@@ -59,6 +59,8 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "fp32", "fp64fast"], default="fp64")
+    ap.add_argument("--indicator", action="store_true",
+                    help="time the form with the learned indicator (weights/indicator_mlp.txt) instead")
     ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -207,6 +209,9 @@ def main():
                                gam="1.6666666666666667", precision=args.precision))
     q0 = H.init_problem(run)
     gpu = H.GpuSolver(H.gpu_params(run))
+    if args.indicator:
+        P, thr = H.load_indicator(WEIGHTS)
+        gpu.set_indicator(P, thr, enable=1)
     gpu.upload_q0(q0)
     gpu.init_faces()
     del q0
@@ -240,7 +245,8 @@ def main():
     d = gpu.diag()
     gpu.close()
 
-    flop_peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
+    flop_peak = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_FP64_TFLOPS
+    plain64 = args.precision == "fp64" and not args.indicator   # the profiled traffic is of this form
     kernels = {}
     for k, (ms, cnt) in tim.items():
         if cnt == 0:
@@ -260,14 +266,14 @@ def main():
         achieved = dk["achieved_tflops"]
         roof = {"bound": "mfma", "achieved": achieved, "peak": flop_peak, "unit": "TFLOP/s",
                 "frac": round(achieved / flop_peak, 4),
-                "traffic": traffic_from_profile(dom),
+                "traffic": traffic_from_profile(dom) if plain64 else None,
                 "kernel": f"k_sweep ({dom}); FP64 vector-ALU bound (no MFMA used: MI355X FP64 vector peak = "
                           f"FP64 matrix peak = 78.6 TFLOP/s)",
                 "per_launch": f"{FLOP_SWEEP_CELL:.0f} FLOP/cell x {zones} cells"}
     else:
         achieved = dk["achieved_gbs"]
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(dom),
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(dom) if plain64 else None,
                 "kernel": dom}
     whole = {
         "fp64_tflops_effective": round(FLOP_ZONE_STEP * value / 1e12, 3),
@@ -288,14 +294,15 @@ def main():
         "dtype": {"fp64": "f64", "fp64fast": "f64", "fp32": "f32 sweeps/CT, f64 stage sums"}[args.precision],
         "data": "synthetic: deterministic mhd_blast3d initial condition",
         "config": {"workload": f"mhd_blast3d {n}^3 periodic, gam 5/3, cour 0.4, p_in 10, p_out 0.1, "
-                               f"B=(1/sqrt2,1/sqrt2,0), plain method, precision {args.precision}",
+                               f"B=(1/sqrt2,1/sqrt2,0), precision {args.precision}, "
+                               + ("learned indicator" if args.indicator else "plain method"),
                    "grid": [n, n, n], "precision": args.precision, "parallelism": "single GPU"},
         "roofline": roof,
         "kernels": kernels,
         "whole_step": whole,
         "divb_ct_max": d.max_divb_ct,
     }
-    if not args.no_forms and args.precision == "fp64":
+    if not args.no_forms and args.precision == "fp64" and not args.indicator:
         forms = {"fp64fast": form_leg(H, torch, run, args, value, "fp64fast"),
                  "fp32": form_leg(H, torch, run, args, value, "fp32")}
         if WEIGHTS.exists():
