@@ -27,11 +27,23 @@ constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = 576, MLP_OFF_W2 = 608, MLP_OFF_B2 = 1
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// fp32 -> bf16, round to nearest even (inputs here are finite)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+// fp32 -> bf16, round to nearest even: gfx950's v_cvt_pk_bf16_f32
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// two values in one conversion: low half a, high half b
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+    const f32x2_t v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+
+// four values as one bf16x4 (64-bit LDS store)
+__device__ __forceinline__ bf16x4 f2bf4(float a, float b, float c, float d) {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 w = {f2bf2(a, b), f2bf2(c, d)};
+    return __builtin_bit_cast(bf16x4, w);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
@@ -223,12 +235,9 @@ __device__ __forceinline__ void layer_relu_t(uint16_t* act, const BFrag& W, cons
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        bf16x4 o;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) o[v] = (short)f2bf(d[u][v] > 0.f ? d[u][v] : 0.f);
-        *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + 16 * u + kq) = o;
-    }
+    for (int u = 0; u < 2; ++u)
+        *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + 16 * u + kq) =
+            f2bf4(fmaxf(d[u][0], 0.f), fmaxf(d[u][1], 0.f), fmaxf(d[u][2], 0.f), fmaxf(d[u][3], 0.f));
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -248,10 +257,8 @@ __device__ __forceinline__ float mlp_forward_wave_t(const float* feat, uint16_t*
         if (grp == t) {
 #pragma unroll
             for (int j = 0; j < MLP_H; j += 4) {
-                bf16x4 o;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) o[v] = (short)((j + v < MLP_IN) ? f2bf(feat[(j + v < MLP_IN) ? j + v : 0]) : 0);
-                *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + j) = o;
+                auto fv = [&](int k) { return k < MLP_IN ? feat[k < MLP_IN ? k : 0] : 0.f; };
+                *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + j) = f2bf4(fv(j), fv(j + 1), fv(j + 2), fv(j + 3));
             }
         }
         __builtin_amdgcn_wave_barrier();
