@@ -52,8 +52,8 @@ constexpr double kFallbackWaveFraction = 0.9;
 constexpr int kFallbackSteps = 8;
 
 // Folds the counter copy of the last routed step into the running totals
-// once it has arrived (wait: block until it has) and takes the fallback
-// decision from it.
+// (wait: block until it has arrived; else only if it has) and takes the
+// fallback decision from it.
 int fold_indicator_stats(howmhd_gpu_ctx* c, bool wait) {
     if (!c->istats_pending) return HOWMHD_OK;
     if (wait) {
@@ -321,7 +321,8 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
     c->route_now = (c->ind_mode == 2);
     if (c->ind_mode == 2) ++c->routed_steps;
     if (c->ind_mode == 1) {
-        const int rc = fold_indicator_stats(c, false);
+        // the last routed step's counts decide this step (waits for that step)
+        const int rc = fold_indicator_stats(c, true);
         if (rc) return rc;
         if (c->fallback_left > 0) {
             --c->fallback_left;

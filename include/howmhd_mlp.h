@@ -64,8 +64,9 @@ HOWMHD_API int howmhd_gpu_indicator_logits(howmhd_gpu_ctx* ctx, int dir, float* 
  *   enable 1: routing with the plain-WENO fallback: each routed step counts
  *             its wavefronts; when at least 90% of them ran the full WENO5
  *             (the network found little to skip), the next 8 steps run the
- *             plain sweep, then one step routes again.  The counts of a step
- *             are read back asynchronously and take effect a step later.
+ *             plain sweep, then one step routes again.  The counts of a
+ *             routed step travel back asynchronously; the next ssprk waits
+ *             for them before it chooses its sweep.
  *   enable 2: routing on every step (no fallback), counted.
  *   enable 0: off (params may be NULL).
  * A context created with indicator HOWMHD_INDICATOR_MLP refuses to step until
