@@ -643,97 +643,65 @@ __device__ __forceinline__ double d4(double om3, double om2, double om1, double 
     return aa * (o0 - om1) + bb * (op1 - om2) + cc * (op2 - om3);
 }
 
-// The face update and the stage combination march along z: a thread owns one
-// (x, y) column of HOWMHD_ZCHUNK cells (block: 64 x-lanes by 4 y-rows), so the
-// z-neighbours of their stencils (6-point D4 and face->centre windows, the dF
-// of the z-sweep at t = z and z + 1) are read once into registers and shifted
-// instead of being fetched again by the blocks of the neighbouring planes.
-// Every cell's arithmetic is the same as with one thread per cell.
-#ifndef HOWMHD_ZCHUNK
-#define HOWMHD_ZCHUNK 16
-#endif
-constexpr int ZLANES = 64, ZROWS = 4;
-
 // fluxct_update_bfield, src/fluxct.cpp:850-877, with the stage initialisation
 // of copy_bfield_stage / prepare_bfield_stage5 (111-135) folded in.
 template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    cell_of(g, i, x, y, z);
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int x = blockIdx.x * ZLANES + (int)(threadIdx.x % ZLANES);
-    const int y = blockIdx.y * ZROWS + (int)(threadIdx.x / ZLANES);
-    if (x >= nx || y >= ny) return;
-    const int zb = blockIdx.z * HOWMHD_ZCHUNK;
-    const int ze = (zb + HOWMHD_ZCHUNK < nz) ? zb + HOWMHD_ZCHUNK : nz;
     const CtFlags& f = A.f;
     const StageCoef& s = A.s;
+    double binit[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const long long o = (long long)c * g.ncell + i;
+        binit[c] = s.stage5 ? A.b0[o] + s.c4 * A.b2[o] + s.c5 * A.b3[o] : A.b0[o];
+    }
     auto Oat = [&](const R* O, int xx, int yy, int zz) -> double { return (double)O[lin(g, xx, yy, zz)]; };
-    // z windows of Oz (bx) and Oy (by): values at z-3 .. z+2
-    const bool wz = f.active && f.use_Oz, wy = f.active && f.use_Oy;
-    double zOz[6] = {0, 0, 0, 0, 0, 0}, zOy[6] = {0, 0, 0, 0, 0, 0};
-    {
+    auto dy_of = [&](const R* O) -> double {
+        const int p = g.per[1];
+        return d4(Oat(O, x, bmap(y - 3, ny, p), z), Oat(O, x, bmap(y - 2, ny, p), z), Oat(O, x, bmap(y - 1, ny, p), z),
+                  Oat(O, x, y, z), Oat(O, x, bmap(y + 1, ny, p), z), Oat(O, x, bmap(y + 2, ny, p), z));
+    };
+    auto dz_of = [&](const R* O) -> double {
         const int p = g.per[2];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int zz = bmap(zb - 3 + k, nz, p);
-            if (wz) zOz[k] = Oat(A.Oz, x, y, zz);
-            if (wy) zOy[k] = Oat(A.Oy, x, y, zz);
+        return d4(Oat(O, x, y, bmap(z - 3, nz, p)), Oat(O, x, y, bmap(z - 2, nz, p)), Oat(O, x, y, bmap(z - 1, nz, p)),
+                  Oat(O, x, y, z), Oat(O, x, y, bmap(z + 1, nz, p)), Oat(O, x, y, bmap(z + 2, nz, p)));
+    };
+    auto dx_of = [&](const R* O) -> double {
+        const int p = g.per[0];
+        return d4(Oat(O, bmap(x - 3, nx, p), y, z), Oat(O, bmap(x - 2, nx, p), y, z), Oat(O, bmap(x - 1, nx, p), y, z),
+                  Oat(O, x, y, z), Oat(O, bmap(x + 1, nx, p), y, z), Oat(O, bmap(x + 2, nx, p), y, z));
+    };
+    double bx = binit[0], by = binit[1], bz = binit[2];
+    if (f.active) {
+        if (f.use_Ox || f.use_Oz) {
+            double dOx = 0.0, dOz = 0.0;
+            if (f.use_Ox) dOx = dy_of(A.Ox);
+            if (f.use_Oz) dOz = dz_of(A.Oz);
+            bx = s.k1 * binit[0] + s.k3 * A.bp[i] - s.dtdy * dOx + s.dtdz * dOz;
+        }
+        if (f.use_Oy || f.use_Ox) {
+            double dOy = 0.0, dOx = 0.0;
+            if (f.use_Oy) dOy = dz_of(A.Oy);
+            if (f.use_Ox) dOx = dx_of(A.Ox);
+            by = s.k1 * binit[1] + s.k3 * A.bp[g.ncell + i] - s.dtdz * dOy + s.dtdx * dOx;
+        }
+        if (f.use_Oz || f.use_Oy) {
+            double dOz = 0.0, dOy = 0.0;
+            if (f.use_Oz) dOz = dx_of(A.Oz);
+            if (f.use_Oy) dOy = dy_of(A.Oy);
+            bz = s.k1 * binit[2] + s.k3 * A.bp[2 * g.ncell + i] - s.dtdx * dOz + s.dtdy * dOy;
         }
     }
-    for (int z = zb; z < ze; ++z) {
-        const long long i = lin(g, x, y, z);
-        double binit[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const long long o = (long long)c * g.ncell + i;
-            binit[c] = s.stage5 ? A.b0[o] + s.c4 * A.b2[o] + s.c5 * A.b3[o] : A.b0[o];
-        }
-        auto dy_of = [&](const R* O) -> double {
-            const int p = g.per[1];
-            return d4(Oat(O, x, bmap(y - 3, ny, p), z), Oat(O, x, bmap(y - 2, ny, p), z),
-                      Oat(O, x, bmap(y - 1, ny, p), z), Oat(O, x, y, z), Oat(O, x, bmap(y + 1, ny, p), z),
-                      Oat(O, x, bmap(y + 2, ny, p), z));
-        };
-        auto dx_of = [&](const R* O) -> double {
-            const int p = g.per[0];
-            return d4(Oat(O, bmap(x - 3, nx, p), y, z), Oat(O, bmap(x - 2, nx, p), y, z),
-                      Oat(O, bmap(x - 1, nx, p), y, z), Oat(O, x, y, z), Oat(O, bmap(x + 1, nx, p), y, z),
-                      Oat(O, bmap(x + 2, nx, p), y, z));
-        };
-        double bx = binit[0], by = binit[1], bz = binit[2];
-        if (f.active) {
-            if (f.use_Ox || f.use_Oz) {
-                double dOx = 0.0, dOz = 0.0;
-                if (f.use_Ox) dOx = dy_of(A.Ox);
-                if (f.use_Oz) dOz = d4(zOz[0], zOz[1], zOz[2], zOz[3], zOz[4], zOz[5]);
-                bx = s.k1 * binit[0] + s.k3 * A.bp[i] - s.dtdy * dOx + s.dtdz * dOz;
-            }
-            if (f.use_Oy || f.use_Ox) {
-                double dOy = 0.0, dOx = 0.0;
-                if (f.use_Oy) dOy = d4(zOy[0], zOy[1], zOy[2], zOy[3], zOy[4], zOy[5]);
-                if (f.use_Ox) dOx = dx_of(A.Ox);
-                by = s.k1 * binit[1] + s.k3 * A.bp[g.ncell + i] - s.dtdz * dOy + s.dtdx * dOx;
-            }
-            if (f.use_Oz || f.use_Oy) {
-                double dOz = 0.0, dOy = 0.0;
-                if (f.use_Oz) dOz = dx_of(A.Oz);
-                if (f.use_Oy) dOy = dy_of(A.Oy);
-                bz = s.k1 * binit[2] + s.k3 * A.bp[2 * g.ncell + i] - s.dtdx * dOz + s.dtdy * dOy;
-            }
-        }
-        A.bo[i] = bx;
-        A.bo[g.ncell + i] = by;
-        A.bo[2 * g.ncell + i] = bz;
-        // shift the z windows to z+1
-        if (z + 1 < ze) {
-            const int zn = bmap(z + 3, nz, g.per[2]);  // the new top of the window
-#pragma unroll
-            for (int k = 0; k < 5; ++k) { zOz[k] = zOz[k + 1]; zOy[k] = zOy[k + 1]; }
-            if (wz) zOz[5] = Oat(A.Oz, x, y, zn);
-            if (wy) zOy[5] = Oat(A.Oy, x, y, zn);
-        }
-    }
+    A.bo[i] = bx;
+    A.bo[g.ncell + i] = by;
+    A.bo[2 * g.ncell + i] = bz;
 }
 
 template <typename R>
@@ -759,20 +727,23 @@ __device__ __forceinline__ double i6(double a, double b, double c, double d, dou
 
 // Stage combination: x_sweep's store q = k1*q + k3*q_prev + dX
 // (src/ssprk.cpp:415-430), then += dY, += dZ (1075-1094, 1166-1185); face ->
-// centre B (fluxct.cpp:882-897); prot (prot.cpp:10-51).  Marches along z
-// (see k_faces): the z-sweep's dF at t = z is the previous cell's dF at t+1,
-// and the face->centre window of b_z shifts by one plane per cell.
+// centre B (fluxct.cpp:882-897); prot (prot.cpp:10-51).
 template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    cell_of(g, i, x, y, z);
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int x = blockIdx.x * ZLANES + (int)(threadIdx.x % ZLANES);
-    const int y = blockIdx.y * ZROWS + (int)(threadIdx.x / ZLANES);
-    if (x >= nx || y >= ny) return;
-    const int zb = blockIdx.z * HOWMHD_ZCHUNK;
-    const int ze = (zb + HOWMHD_ZCHUNK < nz) ? zb + HOWMHD_ZCHUNK : nz;
     const StageCoef& s = A.s;
+    double v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const long long o = (long long)m * g.ncell + i;
+        v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
+    }
     // slot of each q variable in the dF of each sweep (-1: the normal field)
     // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
     // y: rho->0, Mx->3, My->1, Mz->2, Bx->5, By->-, Bz->4, E->6
@@ -780,109 +751,77 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     constexpr int sx[8] = {0, 1, 2, 3, -1, 4, 5, 6};
     constexpr int sy[8] = {0, 3, 1, 2, 5, -1, 4, 6};
     constexpr int sz[8] = {0, 2, 3, 1, 4, 5, -1, 6};
-    const double* bx = A.bo;
-    const double* by = A.bo + g.ncell;
-    const double* bz = A.bo + 2 * g.ncell;
-    const bool wbz = A.f.active && A.f.use_z;
-    // z-sweep fluxes at the left face of the current cell, and the b_z window
-    double fzl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double zbz[6] = {0, 0, 0, 0, 0, 0};
-    if (nz > 1) {
-        const int tl = (zb == 0 && g.per[2]) ? nz : zb;
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-            if (sz[m] >= 0) fzl[m] = (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, zb)];
-    }
-    if (wbz) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) zbz[k] = bz[lin(g, x, y, bmap(zb - 3 + k, nz, g.per[2]))];
-    }
-    for (int z = zb; z < ze; ++z) {
-        const long long i = lin(g, x, y, z);
-        double v[8];
+    if (nx > 1) {
+        const int tl = (x == 0 && g.per[0]) ? nx : x, tr = x + 1;
+        const double dt = s.dtdx;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
-            const long long o = (long long)m * g.ncell + i;
-            v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
-        }
-        if (nx > 1) {
-            const int tl = (x == 0 && g.per[0]) ? nx : x, tr = x + 1;
-            const double dt = s.dtdx;
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                double d = 0.0;
-                if (sx[m] >= 0)
-                    d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] -
-                               (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
-                v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
-            }
-        }
-        if (ny > 1) {
-            const int tl = (y == 0 && g.per[1]) ? ny : y, tr = y + 1;
-            const double dt = s.dtdy;
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                double d = 0.0;
-                if (sy[m] >= 0)
-                    d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] -
-                               (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
-                v[m] = v[m] + d;
-            }
-        }
-        if (nz > 1) {
-            const double dt = s.dtdz;
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                double d = 0.0;
-                if (sz[m] >= 0) {
-                    const double fr = (double)A.dFz[dF_index<2>(g, sz[m], z + 1, x, y, z)];
-                    d = -dt * (fr - fzl[m]);
-                    fzl[m] = fr;
-                }
-                v[m] = v[m] + d;
-            }
-        }
-        if (A.f.active) {
-            if (A.f.use_x) {
-                const int p = g.per[0];
-                v[4] = i6(bx[lin(g, bmap(x - 3, nx, p), y, z)], bx[lin(g, bmap(x - 2, nx, p), y, z)],
-                          bx[lin(g, bmap(x - 1, nx, p), y, z)], bx[i], bx[lin(g, bmap(x + 1, nx, p), y, z)],
-                          bx[lin(g, bmap(x + 2, nx, p), y, z)]);
-            } else {
-                v[4] = bx[i];
-            }
-            if (A.f.use_y) {
-                const int p = g.per[1];
-                v[5] = i6(by[lin(g, x, bmap(y - 3, ny, p), z)], by[lin(g, x, bmap(y - 2, ny, p), z)],
-                          by[lin(g, x, bmap(y - 1, ny, p), z)], by[i], by[lin(g, x, bmap(y + 1, ny, p), z)],
-                          by[lin(g, x, bmap(y + 2, ny, p), z)]);
-            } else {
-                v[5] = by[i];
-            }
-            if (A.f.use_z) {
-                v[6] = i6(zbz[0], zbz[1], zbz[2], zbz[3], zbz[4], zbz[5]);
-            } else {
-                v[6] = bz[i];
-            }
-        }
-        // prot
-        const double gam = A.ph.gam;
-        double rho = fmax(v[0], A.ph.rhomin);
-        const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
-        const double vv2 = vx * vx + vy * vy + vz * vz;
-        const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
-        double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
-        pg = fmax(pg, A.ph.pgmin);
-        v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
-        v[0] = rho;
-#pragma unroll
-        for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
-        if (wbz && z + 1 < ze) {
-#pragma unroll
-            for (int k = 0; k < 5; ++k) zbz[k] = zbz[k + 1];
-            zbz[5] = bz[lin(g, x, y, bmap(z + 3, nz, g.per[2]))];
+            double d = 0.0;
+            if (sx[m] >= 0)
+                d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
+            v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
         }
     }
+    if (ny > 1) {
+        const int tl = (y == 0 && g.per[1]) ? ny : y, tr = y + 1;
+        const double dt = s.dtdy;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            double d = 0.0;
+            if (sy[m] >= 0)
+                d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] - (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
+            v[m] = v[m] + d;
+        }
+    }
+    if (nz > 1) {
+        const int tl = (z == 0 && g.per[2]) ? nz : z, tr = z + 1;
+        const double dt = s.dtdz;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            double d = 0.0;
+            if (sz[m] >= 0)
+                d = -dt * ((double)A.dFz[dF_index<2>(g, sz[m], tr, x, y, z)] - (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, z)]);
+            v[m] = v[m] + d;
+        }
+    }
+    if (A.f.active) {
+        const double* bx = A.bo;
+        const double* by = A.bo + g.ncell;
+        const double* bz = A.bo + 2 * g.ncell;
+        if (A.f.use_x) {
+            const int p = g.per[0];
+            v[4] = i6(bx[lin(g, bmap(x - 3, nx, p), y, z)], bx[lin(g, bmap(x - 2, nx, p), y, z)], bx[lin(g, bmap(x - 1, nx, p), y, z)],
+                      bx[i], bx[lin(g, bmap(x + 1, nx, p), y, z)], bx[lin(g, bmap(x + 2, nx, p), y, z)]);
+        } else {
+            v[4] = bx[i];
+        }
+        if (A.f.use_y) {
+            const int p = g.per[1];
+            v[5] = i6(by[lin(g, x, bmap(y - 3, ny, p), z)], by[lin(g, x, bmap(y - 2, ny, p), z)], by[lin(g, x, bmap(y - 1, ny, p), z)],
+                      by[i], by[lin(g, x, bmap(y + 1, ny, p), z)], by[lin(g, x, bmap(y + 2, ny, p), z)]);
+        } else {
+            v[5] = by[i];
+        }
+        if (A.f.use_z) {
+            const int p = g.per[2];
+            v[6] = i6(bz[lin(g, x, y, bmap(z - 3, nz, p))], bz[lin(g, x, y, bmap(z - 2, nz, p))], bz[lin(g, x, y, bmap(z - 1, nz, p))],
+                      bz[i], bz[lin(g, x, y, bmap(z + 1, nz, p))], bz[lin(g, x, y, bmap(z + 2, nz, p))]);
+        } else {
+            v[6] = bz[i];
+        }
+    }
+    // prot
+    const double gam = A.ph.gam;
+    double rho = fmax(v[0], A.ph.rhomin);
+    const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
+    const double vv2 = vx * vx + vy * vy + vz * vz;
+    const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
+    double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
+    pg = fmax(pg, A.ph.pgmin);
+    v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
+    v[0] = rho;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
 }
 
 }  // namespace howmhd

@@ -32,12 +32,6 @@ inline RK rk_coeff(int iw) {
     return c;
 }
 
-// grid of the z-marching kernels: 64 x 4 (x, y) columns per block, one block
-// per HOWMHD_ZCHUNK planes
-inline dim3 zgrid(const howmhd_gpu_ctx* c) {
-    return dim3(blocks_for(c->g.n[0], ZLANES), blocks_for(c->g.n[1], ZROWS), blocks_for(c->g.n[2], HOWMHD_ZCHUNK));
-}
-
 template <typename R, int MODE>
 void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
     const Geom& g = c->g;
@@ -143,7 +137,7 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     fa.s = s;
     {
         TimedLaunch tl(c, HOWMHD_K_FACES);
-        hipLaunchKernelGGL((k_faces<R, EX>), zgrid(c), dim3(256), 0, c->stream, fa);
+        hipLaunchKernelGGL((k_faces<R, EX>), dim3(nb), dim3(256), 0, c->stream, fa);
     }
     HCHECK(c, hipGetLastError());
 
@@ -163,7 +157,7 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     ca.ph = c->ph;
     {
         TimedLaunch tl(c, HOWMHD_K_COMBINE);
-        hipLaunchKernelGGL((k_combine<R, EX>), zgrid(c), dim3(256), 0, c->stream, ca);
+        hipLaunchKernelGGL((k_combine<R, EX>), dim3(nb), dim3(256), 0, c->stream, ca);
     }
     HCHECK(c, hipGetLastError());
     return HOWMHD_OK;
