@@ -191,3 +191,50 @@ def test_fp32_turbulence_384_stays_close_to_fp64(require_gpu):
     print("turbulence 384^3 FP32 rel L1:", l1, "divB_CT", d32.max_divb_ct)
     assert l1[0] <= 1e-3 and l1[7] <= 1e-3
     assert d32.max_divb_ct <= 1e-10
+
+
+def test_driver_with_indicator_and_fp32(tmp_path, require_gpu):
+    """`how_mhd input.in` with the added keys: precision fp32, indicator mlp
+    and indicator_weights run the routed FP32 form; the final density stays
+    within 1e-3 of the plain FP64 restatement."""
+    weights = H.PKG_DIR / "weights" / "indicator_mlp.txt"
+    inp = tmp_path / "input.in"
+    inp.write_text(f"""problem mhd_blast3d
+output_format vts
+nx 32
+ny 32
+nz 32
+cour 0.4
+tend 0.004
+nt 1
+precision fp32
+indicator mlp
+indicator_weights {weights}
+quiet 1
+""")
+    out = subprocess.run([str(H.DRIVER), str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "indicator weights" in out.stdout and "precision          = fp32" in out.stdout
+    files = sorted((tmp_path / "output").glob("dump*_rank000000.vts"))
+    assert len(files) == 2
+    run = H.parse(inp.read_text())
+    p = H.gpu_params(run)
+    p.precision = H.PREC["fp64"]
+    p.indicator = H.INDICATOR["off"]
+    ref = O.RefSolver(p)
+    ref.upload_q0(H.init_problem(run))
+    ref.init_faces()
+    t = 0.0
+    while t < run.tend:
+        dt = min(ref.tstep(), run.tend - t)
+        ref.ssprk(dt)
+        t += dt
+    raw = files[-1].read_bytes()
+    head, _, tail = raw.partition(b'<AppendedData encoding="raw">\n_')
+    off = int(re.search(rb'Name="rho" NumberOfComponents="1" format="appended" offset="(\d+)"', head).group(1))
+    nbytes = struct.unpack_from("<Q", tail, off)[0]
+    rho = np.frombuffer(tail, dtype="<f8", count=nbytes // 8, offset=off + 8).reshape(32, 32, 32)
+    want = H.active(ref.q0())[0].transpose(2, 1, 0)
+    l1 = np.abs(rho - want).sum() / np.abs(want).sum()
+    print("driver fp32+indicator vs FP64 restatement, density rel L1:", l1)
+    assert l1 <= 1e-3
