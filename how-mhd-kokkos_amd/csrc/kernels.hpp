@@ -195,56 +195,11 @@ __device__ __forceinline__ void opaque(Eig<R>& o) {
     opq(o.sqrtrho); opq(o.invsqrtrho); opq(o.sgnBt); opq(o.nrm_half); opq(o.nrm_ent);
 }
 
-// a[m] for a wave index known only at run time, without indexing a private
-// array (which would live in scratch memory).
+// The seven per-wave LLF speeds, accumulated as named registers.
 template <typename R>
 struct Seven {
     R v0, v1, v2, v3, v4, v5, v6;
 };
-template <typename R>
-__device__ __forceinline__ R pick7(const Seven<R>& a, int m) {
-    R v;
-    switch (m) {  // scalar branch; the asm keeps it from becoming an indexed load
-    case 0: asm volatile(""); v = a.v0; break;
-    case 1: asm volatile(""); v = a.v1; break;
-    case 2: asm volatile(""); v = a.v2; break;
-    case 3: asm volatile(""); v = a.v3; break;
-    case 4: asm volatile(""); v = a.v4; break;
-    case 5: asm volatile(""); v = a.v5; break;
-    default: asm volatile(""); v = a.v6; break;
-    }
-    return v;
-}
-
-// Row m of L and column m of R for a wave index known only at run time (the
-// same for the whole wavefront, so the switch is a scalar branch).  The empty
-// volatile asm in each case keeps the compiler from evaluating all seven
-// cases and selecting one.
-template <typename R>
-__device__ __forceinline__ void l_row_dyn(int m, R gam, const Eig<R>& e, R* L) {
-    switch (m) {
-    case 0: asm volatile(""); l_row<0>(gam, e, L); break;
-    case 1: asm volatile(""); l_row<1>(gam, e, L); break;
-    case 2: asm volatile(""); l_row<2>(gam, e, L); break;
-    case 3: asm volatile(""); l_row<3>(gam, e, L); break;
-    case 4: asm volatile(""); l_row<4>(gam, e, L); break;
-    case 5: asm volatile(""); l_row<5>(gam, e, L); break;
-    default: asm volatile(""); l_row<6>(gam, e, L); break;
-    }
-}
-template <typename R>
-__device__ __forceinline__ void r_col_dyn(int m, R gam, const Eig<R>& e, R* Rc) {
-    switch (m) {
-    case 0: asm volatile(""); r_col<0>(gam, e, Rc); break;
-    case 1: asm volatile(""); r_col<1>(gam, e, Rc); break;
-    case 2: asm volatile(""); r_col<2>(gam, e, Rc); break;
-    case 3: asm volatile(""); r_col<3>(gam, e, Rc); break;
-    case 4: asm volatile(""); r_col<4>(gam, e, Rc); break;
-    case 5: asm volatile(""); r_col<5>(gam, e, Rc); break;
-    default: asm volatile(""); r_col<6>(gam, e, Rc); break;
-    }
-}
-
 template <typename R, int DIR, int MODE, int EX>
 __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");

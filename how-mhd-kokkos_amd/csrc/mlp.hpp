@@ -135,79 +135,6 @@ __device__ __forceinline__ void mlp_features(const float* rho, const float* p, c
     }
 }
 
-// Forward pass of the wavefront: this lane's 18 features in, its output
-// logit z3 back.  `act` is this wavefront's 64 x MLP_ROW bf16 scratch.
-__device__ __forceinline__ float mlp_forward_wave(const float* feat, uint16_t* act, const BFrag& W1, const BFrag& W2,
-                                                  const MlpDev& m) {
-    const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int j = 0; j < MLP_H; ++j) act[l * MLP_ROW + j] = (j < MLP_IN) ? f2bf(feat[j < MLP_IN ? j : 0]) : 0;
-    __builtin_amdgcn_wave_barrier();
-    layer_relu(act, act, W1, m.b1);
-    layer_relu(act, act, W2, m.b2);
-    float z = m.b3;
-#pragma unroll
-    for (int j = 0; j < MLP_H; ++j) z += bf2f(act[l * MLP_ROW + j]) * m.w3[j];
-    return z;
-}
-
-// The same forward pass with a 16-row scratch per wavefront (16 x MLP_ROW
-// bf16): the four 16-lane groups run through both layers one after another
-// (per sample the arithmetic is the same as mlp_forward_wave).  The sweep
-// uses this form to keep its LDS footprint small.
-__device__ __forceinline__ void layer_relu16(uint16_t* act, const BFrag& W, const float* bias) {
-    const int l = threadIdx.x & 63;
-    const int r16 = l % 16, kq = 4 * (l / 16);
-    bf16x4 a[2];
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) a[s][v] = (short)act[r16 * MLP_ROW + 16 * s + kq + v];
-    f32x4 d[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const float b = bias[16 * u + r16];
-        f32x4 c = {b, b, b, b};
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], W.f[0][u], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], W.f[1][u], c, 0, 0, 0);
-        d[u] = c;
-    }
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const float z = d[u][v];
-            act[(kq + v) * MLP_ROW + 16 * u + r16] = f2bf(z > 0.f ? z : 0.f);
-        }
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ float mlp_forward_wave16(const float* feat, uint16_t* act, const BFrag& W1,
-                                                    const BFrag& W2, const MlpDev& m) {
-    const int l = threadIdx.x & 63;
-    const int grp = l >> 4, r16 = l & 15;
-    float zl = 0.f;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        if (grp == t) {
-#pragma unroll
-            for (int j = 0; j < MLP_H; ++j) act[r16 * MLP_ROW + j] = (j < MLP_IN) ? f2bf(feat[j < MLP_IN ? j : 0]) : 0;
-        }
-        __builtin_amdgcn_wave_barrier();
-        layer_relu16(act, W1, m.b1);
-        layer_relu16(act, W2, m.b2);
-        if (grp == t) {
-            float z = m.b3;
-#pragma unroll
-            for (int j = 0; j < MLP_H; ++j) z += bf2f(act[r16 * MLP_ROW + j]) * m.w3[j];
-            zl = z;
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    return zl;
-}
-
 // The forward pass the routed sweep uses.  Transposed products: each layer is
 // D = W A^T, with the weights as the MFMA A operand (the B fragments of
 // load_bfrag hold exactly W[16u + l%16][16s + 4(l/16) + v]) and the
