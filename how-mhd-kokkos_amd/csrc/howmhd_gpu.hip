@@ -523,16 +523,25 @@ int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float thres
                 return fail(c, HOWMHD_EINVAL, "howmhd_gpu_set_indicator: non-finite weight " + std::to_string(j));
         MlpDev h;
         std::memset(&h, 0, sizeof(h));
-        for (int k = 0; k < MLP_H; ++k)
-            for (int nn = 0; nn < MLP_H; ++nn) {
-                h.W1t[k][nn] = k < MLP_IN ? bf16_bits(params[MLP_OFF_W1 + nn * MLP_IN + k]) : 0;
-                h.W2t[k][nn] = bf16_bits(params[MLP_OFF_W2 + nn * MLP_H + k]);
-            }
-        for (int j = 0; j < MLP_H; ++j) {
-            h.b1[j] = params[MLP_OFF_B1 + j];
-            h.b2[j] = params[MLP_OFF_B2 + j];
-            h.w3[j] = bf16_round(params[MLP_OFF_W3 + j]);
+        // A fragments of the sweep's 32x32x16 products (mlp_forward_wave):
+        // lane l holds row m = l % 32 and k-slots v of half-wave hw = l / 32
+        for (int l = 0; l < 64; ++l) {
+            const int m = l % 32, hw = l / 32;
+            for (int s = 0; s < 2; ++s)
+                for (int v = 0; v < 8; ++v) {
+                    // layer 1: k-step 0 slot (hw, v) = feature 8 hw + v; k-step 1 slot (0, v) = 16 + v
+                    const int f = s == 0 ? 8 * hw + v : (hw == 0 ? 16 + v : MLP_IN);
+                    h.W1f[l][s][v] = f < MLP_IN ? bf16_bits(params[MLP_OFF_W1 + m * MLP_IN + f]) : 0;
+                    const int j = mlp_hidden_slot(s, v, hw);
+                    h.W2f[l][s][v] = bf16_bits(params[MLP_OFF_W2 + m * MLP_H + j]);
+                    h.W3f[l][s][v] = m == 0 ? bf16_bits(params[MLP_OFF_W3 + j]) : 0;
+                }
         }
+        for (int hw = 0; hw < 2; ++hw)
+            for (int r = 0; r < 16; ++r) {
+                h.b1r[hw][r] = params[MLP_OFF_B1 + mlp_acc_row(r, hw)];
+                h.b2r[hw][r] = params[MLP_OFF_B2 + mlp_acc_row(r, hw)];
+            }
         h.b3 = params[MLP_OFF_B3];
         h.logit_threshold = std::log(threshold / (1.f - threshold));
         HCHECK(c, hipMemcpy(c->mlp, &h, sizeof(h), hipMemcpyHostToDevice));

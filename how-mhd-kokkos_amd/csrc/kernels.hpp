@@ -200,13 +200,22 @@ template <typename R>
 struct Seven {
     R v0, v1, v2, v3, v4, v5, v6;
 };
+// Wavefronts per SIMD the sweep is compiled for: FP32 routed sweeps 4 (at
+// most 128 registers; four blocks' tiles fill the 160 KB of LDS), the others
+// HOWMHD_SWEEP_WAVES.
+#ifndef HOWMHD_ROUTED_WAVES
+#define HOWMHD_ROUTED_WAVES 4
+#endif
+template <typename R, int MODE>
+constexpr int sweep_waves() { return (sizeof(R) == 4 && MODE == SW_ROUTED) ? HOWMHD_ROUTED_WAVES : HOWMHD_SWEEP_WAVES; }
+
 template <typename R, int DIR, int MODE, int EX>
-__global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, MODE>()))) void k_sweep(
+    SweepArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
     using T = SweepTile<DIR>;
     __shared__ UF<R> luf[7][T::CELLS];
     __shared__ R lo[O_COUNT][T::CELLS];
-    __shared__ __attribute__((aligned(16))) uint16_t mlp_act[MODE != SW_PLAIN ? 4 : 1][MODE != SW_PLAIN ? 16 * MLP_ROWT : 1];
     // |B| of each cell for the indicator features (routed and sample sweeps)
     __shared__ float lbmag[MODE != SW_PLAIN ? T::CELLS : 1];
     const Geom& g = A.g;
@@ -299,20 +308,14 @@ __global__ __launch_bounds__(256, HOWMHD_SWEEP_WAVES) void k_sweep(SweepArgs<R> 
         if (A.logit) {   // uniform over the launch
             float f[MLP_IN];
             features(f);
-            BFrag W1, W2;
-            load_bfrag(A.mlp->W1t, W1);
-            load_bfrag(A.mlp->W2t, W2);
-            const float zl = mlp_forward_wave_t(f, mlp_act[tid / 64], W1, W2, *A.mlp);
+            const float zl = mlp_forward_wave(f, *A.mlp);
             if (valid) A.logit[sample_index<DIR>(g, t, x, y, z)] = zl;
         }
     }
     if constexpr (MODE == SW_ROUTED) {
         float f[MLP_IN];
         features(f);
-        BFrag W1, W2;
-        load_bfrag(A.mlp->W1t, W1);
-        load_bfrag(A.mlp->W2t, W2);
-        const float zl = mlp_forward_wave_t(f, mlp_act[tid / 64], W1, W2, *A.mlp);
+        const float zl = mlp_forward_wave(f, *A.mlp);
         flagged = valid && (zl > A.mlp->logit_threshold);
         if (A.stats) {
             const unsigned long long fb = __ballot(flagged), vb = __ballot(valid);

@@ -566,23 +566,18 @@ __device__ __forceinline__ R weno_wave_dev(const R* Fsk, const R* qsk, R amx, R&
 }
 
 // Linear upwind-5 flux of one component with one LLF speed alpha: the WENO5
-// flux of weno_wave with the ideal weights (w0 = 0.1, w2 = 0.3), where both
-// corrections reduce to (a - 3b + 3c - d)/30.  With one alpha for all waves the
-// characteristic projection cancels (R L = I), so it runs on the physical
-// components directly.
+// flux of weno_wave with the ideal weights (w0 = 0.1, w2 = 0.3).  Expanded in
+// the cell values it is
+//   (F0 - 8 F1 + 37 F2 + 37 F3 - 8 F4 + F5 + alpha (U0 - 5 U1 + 10 U2 - 10 U3 + 5 U4 - U5)) / 60,
+// a centred 6th-order flux plus the 5th-difference LLF dissipation, taken
+// here in symmetric pairs.  With one alpha for all waves the characteristic
+// projection cancels (R L = I), so it runs on the physical components
+// directly.
 template <typename R>
 __device__ __forceinline__ R upwind5(const R* F, const R* U, R alpha) {
-    R dF[5], dU[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        dF[k] = F[k + 1] - F[k];
-        dU[k] = U[k + 1] - U[k];
-    }
-    const R first = (-F[1] + R(7) * F[2] + R(7) * F[3] - F[4]) * R(1.0 / 12.0);
-    const R a = dF[0] + alpha * dU[0], b = dF[1] + alpha * dU[1], c = dF[2] + alpha * dU[2], d = dF[3] + alpha * dU[3];
-    const R e = dF[4] - alpha * dU[4], f = dF[3] - alpha * dU[3], g = dF[2] - alpha * dU[2], h = dF[1] - alpha * dU[1];
-    // 0.5 (a - 3b + 3c - d)/30 for each side
-    return first - (a - R(3) * b + R(3) * c - d) * R(1.0 / 60.0) + (e - R(3) * f + R(3) * g - h) * R(1.0 / 60.0);
+    const R sf = (F[0] + F[5]) - R(8) * (F[1] + F[4]) + R(37) * (F[2] + F[3]);
+    const R su = (U[0] - U[5]) - R(5) * (U[1] - U[4]) + R(10) * (U[2] - U[3]);
+    return (sf + alpha * su) * R(1.0 / 60.0);
 }
 
 }  // namespace howmhd

@@ -47,31 +47,33 @@ __device__ __forceinline__ bf16x4 f2bf4(float a, float b, float c, float d) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
-// Device form of the weights for inference: bf16 B operands of layers 1 and
-// 2 (transposed, layer-1 k padded 18 -> 32 with zeros), FP32 biases, bf16
-// output weights held as floats.
+// Device form of the weights for the sweep's network: per-lane A fragments of
+// the three products (MFMA 32x32x16 bf16, two k-steps each, see
+// mlp_forward_wave), the FP32 biases of layers 1 and 2 in accumulator-row
+// order, and the FP32 output bias.
 struct MlpDev {
-    uint16_t W1t[MLP_H][MLP_H];  // W1t[k][n] = W1[n][k], k < 18
-    uint16_t W2t[MLP_H][MLP_H];  // W2t[k][n] = W2[n][k]
-    float b1[MLP_H], b2[MLP_H], w3[MLP_H];
+    alignas(16) uint16_t W1f[64][2][8];
+    alignas(16) uint16_t W2f[64][2][8];
+    alignas(16) uint16_t W3f[64][2][8];
+    alignas(16) float b1r[2][16];
+    alignas(16) float b2r[2][16];
     float b3;
     float logit_threshold;       // flag when z3 > threshold (0 <-> p > 0.5)
 };
+
+// Output unit held in accumulator row r (0..15) by the lanes of half-wave h
+// (lanes 32h..32h+31) of a 32x32 MFMA result; the lane's column is the
+// sample l % 32.
+__host__ __device__ constexpr int mlp_acc_row(int r, int h) { return 8 * (r / 4) + 4 * h + r % 4; }
+// Input unit of k-slot v (0..7) of k-step s seen by half-wave h in layers 2
+// and 3, whose B operand is the previous layer's accumulator as it lies in
+// the lane (rows 8s..8s+7).
+__host__ __device__ constexpr int mlp_hidden_slot(int s, int v, int h) { return mlp_acc_row(8 * s + v, h); }
 
 // B-operand fragments of one 32x32 bf16 matrix Wt[k][n]: [k-step s][n-tile u]
 struct BFrag {
     bf16x4 f[2][2];
 };
-
-__device__ __forceinline__ void load_bfrag(const uint16_t (*Wt)[MLP_H], BFrag& b) {
-    const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) b.f[s][u][v] = (short)Wt[16 * s + 4 * (l / 16) + v][16 * u + (l % 16)];
-}
 
 // Scratch rows of one wavefront: 64 rows x 32 bf16 (+2 pad to stagger banks).
 constexpr int MLP_ROW = MLP_H + 2;
@@ -135,73 +137,88 @@ __device__ __forceinline__ void mlp_features(const float* rho, const float* p, c
     }
 }
 
-// The forward pass the routed sweep uses.  Transposed products: each layer is
-// D = W A^T, with the weights as the MFMA A operand (the B fragments of
-// load_bfrag hold exactly W[16u + l%16][16s + 4(l/16) + v]) and the
-// activations of 16 samples as the B operand, so that every lane reads and
-// writes its activations as 4 consecutive bf16 (64-bit LDS accesses); the
-// output layer is a third product whose A operand is w3 in row 0, and the
-// logits are moved to their lanes with a cross-lane permute.
-// act: 16 x MLP_ROWT bf16 per wavefront (8-byte aligned rows).
-constexpr int MLP_ROWT = 36;
+// The forward pass of the sweeps (routing and in-sweep logits): the 64
+// lanes' interfaces are two 32-sample tiles, each layer is one pair of
+// v_mfma_f32_32x32x16_bf16 products D = W X^T (output units x samples), and
+// everything stays in registers.
+//   * Layer-1 operand X^T: k-slot (h, v) of k-step 0 is feature 8h + v, of
+//     k-step 1 feature 16 + v (h = 0, v < 2; the rest is 0).  Lane l holds
+//     its own interface's features, so the half-waves trade halves with
+//     v_permlane32_swap: one swap yields the operands of both tiles.
+//   * A 32x32 result leaves output unit mlp_acc_row(r, h) of sample l % 32 in
+//     row r of lane l.  The next layer takes these 16 rows, ReLU'd and
+//     rounded to bf16, as its B operand unchanged (rows 0..7: k-step 0, rows
+//     8..15: k-step 1); the weights' A fragments follow the same k order
+//     (mlp_hidden_slot), so no data moves between layers.
+//   * The output layer's A operand is w3 in row 0: z3 - b3 of sample l of the
+//     tile lands in row 0 of lane l < 32, and one more swap returns each
+//     tile's logits to the lanes of its interfaces.
+// Per sample the products are the standalone inference kernel's (bf16
+// operands, FP32 accumulation); only the order of the FP32 sums differs.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ bf16x4 lds_b64(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
-
-__device__ __forceinline__ void layer_relu_t(uint16_t* act, const BFrag& W, const float* bias) {
-    const int l = threadIdx.x & 63;
-    const int r16 = l % 16, kq = 4 * (l / 16);
-    const bf16x4 b0 = lds_b64(act + r16 * MLP_ROWT + kq);
-    const bf16x4 b1 = lds_b64(act + r16 * MLP_ROWT + 16 + kq);
-    f32x4 d[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        f32x4 c = {bias[16 * u + kq], bias[16 * u + kq + 1], bias[16 * u + kq + 2], bias[16 * u + kq + 3]};
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W.f[0][u], b0, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W.f[1][u], b1, c, 0, 0, 0);
-        d[u] = c;
-    }
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-        *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + 16 * u + kq) =
-            f2bf4(fmaxf(d[u][0], 0.f), fmaxf(d[u][1], 0.f), fmaxf(d[u][2], 0.f), fmaxf(d[u][3], 0.f));
-    __builtin_amdgcn_wave_barrier();
+// bf16(max(a, 0)), bf16(max(b, 0)) as one dword: rounding keeps the sign, so
+// the ReLU is an integer max of the bf16 patterns (negative and -0 -> +0)
+__device__ __forceinline__ uint32_t relu_bf2(float a, float b) {
+    const s16x2 v = __builtin_bit_cast(s16x2, f2bf2(a, b));
+    const s16x2 z = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, z));
 }
 
-__device__ __forceinline__ float mlp_forward_wave_t(const float* feat, uint16_t* act, const BFrag& W1,
-                                                    const BFrag& W2, const MlpDev& m) {
+__device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+__device__ __forceinline__ void relu_pack(const f32x16& c, u32x4& lo, u32x4& hi) {
+    lo = u32x4{relu_bf2(c[0], c[1]), relu_bf2(c[2], c[3]), relu_bf2(c[4], c[5]), relu_bf2(c[6], c[7])};
+    hi = u32x4{relu_bf2(c[8], c[9]), relu_bf2(c[10], c[11]), relu_bf2(c[12], c[13]), relu_bf2(c[14], c[15])};
+}
+
+__device__ __forceinline__ float mlp_forward_wave(const float* feat, const MlpDev& m) {
     const int l = threadIdx.x & 63;
-    const int grp = l >> 4, r16 = l & 15, kq = 4 * (l >> 4);
-    // w3 as the A operand: row 0 only
-    bf16x4 w3f[2];
+    const int h = l >> 5;
+    uint32_t P[9];
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int d = 0; d < 9; ++d) P[d] = f2bf2(feat[2 * d], feat[2 * d + 1]);
+    u32x4 x0[2], x1[2];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) w3f[s][v] = (short)(r16 == 0 ? f2bf(m.w3[16 * s + kq + v]) : 0);
-    float zl = 0.f;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        if (grp == t) {
-#pragma unroll
-            for (int j = 0; j < MLP_H; j += 4) {
-                auto fv = [&](int k) { return k < MLP_IN ? feat[k < MLP_IN ? k : 0] : 0.f; };
-                *reinterpret_cast<bf16x4*>(act + r16 * MLP_ROWT + j) = f2bf4(fv(j), fv(j + 1), fv(j + 2), fv(j + 3));
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        layer_relu_t(act, W1, m.b1);
-        layer_relu_t(act, W2, m.b2);
-        const bf16x4 b0 = lds_b64(act + r16 * MLP_ROWT + kq);
-        const bf16x4 b1 = lds_b64(act + r16 * MLP_ROWT + 16 + kq);
-        f32x4 c = {kq == 0 ? m.b3 : 0.f, 0.f, 0.f, 0.f};
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3f[0], b0, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3f[1], b1, c, 0, 0, 0);
-        // lanes 0..15 hold z3 of samples 0..15 of this tile in c[0]
-        const float z = __shfl(c[0], r16, 64);
-        if (grp == t) zl = z;
-        __builtin_amdgcn_wave_barrier();
+    for (int d = 0; d < 4; ++d) {
+        const auto r = __builtin_amdgcn_permlane32_swap(P[d], P[4 + d], false, false);
+        x0[0][d] = r[0];
+        x0[1][d] = r[1];
     }
-    return zl;
+    {
+        const auto r = __builtin_amdgcn_permlane32_swap(P[8], 0u, false, false);
+        x1[0] = u32x4{r[0], 0u, 0u, 0u};
+        x1[1] = u32x4{r[1], 0u, 0u, 0u};
+    }
+    const u32x4* w1 = reinterpret_cast<const u32x4*>(&m.W1f[l][0][0]);
+    const u32x4* w2 = reinterpret_cast<const u32x4*>(&m.W2f[l][0][0]);
+    const u32x4* w3 = reinterpret_cast<const u32x4*>(&m.W3f[l][0][0]);
+    const f32x16 b1 = *reinterpret_cast<const f32x16*>(&m.b1r[h][0]);
+    const f32x16 b2 = *reinterpret_cast<const f32x16*>(&m.b2r[h][0]);
+    const u32x4 w1a = w1[0], w1b = w1[1], w2a = w2[0], w2b = w2[1], w3a = w3[0], w3b = w3[1];
+    float z[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        f32x16 c = mfma32(w1a, x0[t], b1);
+        c = mfma32(w1b, x1[t], c);
+        u32x4 lo, hi;
+        relu_pack(c, lo, hi);
+        c = mfma32(w2a, lo, b2);
+        c = mfma32(w2b, hi, c);
+        relu_pack(c, lo, hi);
+        f32x16 o = {};
+        o = mfma32(w3a, lo, o);
+        o = mfma32(w3b, hi, o);
+        z[t] = o[0];
+    }
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(z[0]), __float_as_uint(z[1]), false, false);
+    return __uint_as_float(r[0]) + m.b3;
 }
 
 }  // namespace howmhd

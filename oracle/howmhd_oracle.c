@@ -550,22 +550,14 @@ static double weno_wave(const double* Fsk, const double* qsk, double amx, double
 /* Linear upwind-5 flux of one component with one LLF speed alpha (the
  * interface flux of the learned-indicator form for unflagged interfaces; no
  * counterpart in the reference).  It is weno_wave with the ideal weights
- * (0.1, 0.6, 0.3), where both corrections reduce to (a - 3b + 3c - d)/30 on
- * half-differences; with one alpha for all waves the characteristic
- * projection cancels, so it acts on the components directly.  Evaluation
- * order as the HIP kernel. */
+ * (0.1, 0.6, 0.3), expanded in the cell values:
+ * (F0 - 8F1 + 37F2 + 37F3 - 8F4 + F5 + alpha (U0 - 5U1 + 10U2 - 10U3 + 5U4 - U5))/60;
+ * with one alpha for all waves the characteristic projection cancels, so it
+ * acts on the components directly.  Evaluation order as the HIP kernel. */
 static double upwind5(const double* F, const double* U, double alpha) {
-    double dF[5], dU[5];
-    for (int k = 0; k < 5; ++k) {
-        dF[k] = F[k + 1] - F[k];
-        dU[k] = U[k + 1] - U[k];
-    }
-    const double first = (-F[1] + 7.0 * F[2] + 7.0 * F[3] - F[4]) * (1.0 / 12.0);
-    const double a = dF[0] + alpha * dU[0], b = dF[1] + alpha * dU[1];
-    const double c = dF[2] + alpha * dU[2], d = dF[3] + alpha * dU[3];
-    const double e = dF[4] - alpha * dU[4], f = dF[3] - alpha * dU[3];
-    const double g = dF[2] - alpha * dU[2], h = dF[1] - alpha * dU[1];
-    return first - (a - 3.0 * b + 3.0 * c - d) * (1.0 / 60.0) + (e - 3.0 * f + 3.0 * g - h) * (1.0 / 60.0);
+    const double sf = (F[0] + F[5]) - 8.0 * (F[1] + F[4]) + 37.0 * (F[2] + F[3]);
+    const double su = (U[0] - U[5]) - 5.0 * (U[1] - U[4]) + 10.0 * (U[2] - U[3]);
+    return (sf + alpha * su) * (1.0 / 60.0);
 }
 
 /* slot of conserved component j (rho, Mn, Mt1, Mt2, Bt1, Bt2, E) in qx */
