@@ -63,9 +63,8 @@ int fold_indicator_stats(howmhd_gpu_ctx* c, bool wait) {
         if (q == hipErrorNotReady) return HOWMHD_OK;
         HCHECK(c, q);
     }
-    long long s[4] = {0, 0, 0, 0};
-    for (int i = 0; i < 64; ++i)
-        for (int k = 0; k < 4; ++k) s[k] += (long long)c->istats_host[4 * i + k];
+    long long s[4];
+    for (int k = 0; k < 4; ++k) s[k] = (long long)c->istats_host[k];
     for (int k = 0; k < 4; ++k) c->acc[k] += s[k];
     c->istats_pending = false;
     if (s[3] > 0 && (double)s[2] >= kFallbackWaveFraction * (double)s[3]) c->fallback_left = kFallbackSteps;
@@ -217,8 +216,14 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     }
     ok = ok && alloc((void**)&c->dmax, 4 * sizeof(unsigned long long));
     ok = ok && alloc((void**)&c->mlp, sizeof(MlpDev));
-    ok = ok && alloc((void**)&c->istats, 64 * 4 * sizeof(unsigned long long));
-    ok = ok && hipHostMalloc((void**)&c->istats_host, 64 * 4 * sizeof(unsigned long long), hipHostMallocDefault) ==
+    // one counter slot per wavefront of the largest sweep grid
+    for (int d = 0; d < 3; ++d) {
+        const dim3 gr = sweep_grid(c->g, d);
+        c->nwave = std::max(c->nwave, 4LL * gr.x * gr.y * gr.z);
+    }
+    ok = ok && alloc((void**)&c->iwave, (size_t)c->nwave * sizeof(uint4));
+    ok = ok && alloc((void**)&c->istats, 4 * sizeof(unsigned long long));
+    ok = ok && hipHostMalloc((void**)&c->istats_host, 4 * sizeof(unsigned long long), hipHostMallocDefault) ==
                    hipSuccess;
     ok = ok && hipEventCreateWithFlags(&c->istats_ev, hipEventDisableTiming) == hipSuccess;
     if (!ok) return cleanup_fail(HOWMHD_ENOMEM, "howmhd_gpu_create: device allocation failed");
@@ -243,6 +248,7 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
     if (c->dmax) (void)hipFree(c->dmax);
     if (c->mlp) (void)hipFree(c->mlp);
     if (c->istats) (void)hipFree(c->istats);
+    if (c->iwave) (void)hipFree(c->iwave);
     if (c->istats_host) (void)hipHostFree(c->istats_host);
     if (c->istats_ev) (void)hipEventDestroy(c->istats_ev);
     drain_timing(c);
@@ -336,11 +342,17 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
         const int rc = stage(c, iw, dt);
         if (rc) return rc;
     }
+    if (c->route_now) {
+        // the step's per-wavefront counters into the totals (slots cleared)
+        const unsigned nb = std::min(1024u, blocks_for((size_t)c->nwave, 256));
+        hipLaunchKernelGGL(k_fold_wave_stats, dim3(nb), dim3(256), 0, c->stream, c->iwave, c->nwave, c->istats);
+        HCHECK(c, hipGetLastError());
+    }
     if (c->ind_mode == 1 && c->route_now && !c->istats_pending) {
-        // this step's counters to the host, then clear them for the next
-        HCHECK(c, hipMemcpyAsync(c->istats_host, c->istats, 64 * 4 * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, c->stream));
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, c->istats, 256);
+        // this step's totals to the host, then clear them for the next
+        HCHECK(c, hipMemcpyAsync(c->istats_host, c->istats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 c->stream));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->istats, 4);
         HCHECK(c, hipGetLastError());
         HCHECK(c, hipEventRecord(c->istats_ev, c->stream));
         c->istats_pending = true;
@@ -546,7 +558,8 @@ int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float thres
         h.logit_threshold = std::log(threshold / (1.f - threshold));
         HCHECK(c, hipMemcpy(c->mlp, &h, sizeof(h), hipMemcpyHostToDevice));
     }
-    HCHECK(c, hipMemset(c->istats, 0, 64 * 4 * sizeof(unsigned long long)));
+    HCHECK(c, hipMemset(c->istats, 0, 4 * sizeof(unsigned long long)));
+    HCHECK(c, hipMemset(c->iwave, 0, (size_t)c->nwave * sizeof(uint4)));
     c->istats_pending = false;
     c->fallback_left = 0;
     for (int k = 0; k < 4; ++k) c->acc[k] = 0;
@@ -561,12 +574,11 @@ int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* c, int64_t* flagged, int64_t* tot
     HCHECK(c, hipStreamSynchronize(c->stream));
     const int rc = fold_indicator_stats(c, true);
     if (rc) return rc;
-    unsigned long long h[64 * 4];
+    unsigned long long h[4];
     HCHECK(c, hipMemcpy(h, c->istats, sizeof(h), hipMemcpyDeviceToHost));
     HCHECK(c, hipMemset(c->istats, 0, sizeof(h)));
     long long s[4] = {c->acc[0], c->acc[1], c->acc[2], c->acc[3]};
-    for (int i = 0; i < 64; ++i)
-        for (int k = 0; k < 4; ++k) s[k] += (long long)h[4 * i + k];
+    for (int k = 0; k < 4; ++k) s[k] += (long long)h[k];
     for (int k = 0; k < 4; ++k) c->acc[k] = 0;
     *flagged = s[0];
     *total = s[1];

@@ -112,10 +112,13 @@ struct SweepArgs {
     float* feat;
     float* label;
     float* logit;
-    // SW_ROUTED: device weights; stats (null: not counted) holds 64 slots of
-    // (flagged, total, wavefronts flagged, wavefronts total), summed by the host
+    // SW_ROUTED: device weights; stats (null: not counted): one slot per
+    // wavefront of the launch grid (block-linear index x 4 + wavefront), to
+    // which the wavefront adds (flagged, total, 1 if any flagged, 1): plain
+    // read-modify-writes (every slot has one writer per launch) that
+    // k_fold_wave_stats sums and clears after the step
     const MlpDev* mlp;
-    unsigned long long* stats;
+    uint4* stats;
 };
 
 // dF layout per direction: component-major, interface index t = 0..n along the
@@ -237,8 +240,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
         else y0 = blockIdx.z;
     }
 
-    // ---- phase 1: cell values of the tile (stencil halo included) into LDS
-    for (int s = tid; s < T::CELLS; s += 256) {
+    // ---- phase 1: cell values of the tile (stencil halo included) into LDS.
+    // A thread's cells are tid, tid + 256, ...; all their loads are issued
+    // before the first conversion, so a block waits on memory once.
+    auto cell_index = [&](int s) -> long long {
         int cx, cy, cz, along;
         if constexpr (DIR == 0) {
             const int row = s / (T::LANES + 5), c = s % (T::LANES + 5);
@@ -258,18 +263,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
         if constexpr (DIR == 0) cx = a;
         else if constexpr (DIR == 1) cy = a;
         else cz = a;
-        const long long idx = ((long long)cz * g.n[1] + cy) * g.n[0] + cx;
-        double qs[8];
+        return ((long long)cz * g.n[1] + cy) * g.n[0] + cx;
+    };
+    constexpr int NIT = (T::CELLS + 255) / 256;
+    double qs[NIT][8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) qs[m] = A.q[(long long)kPerm[DIR][m] * g.ncell + idx];
-        R out[C_NFIELD];
-        cell_values<R>(qs, A.ph.gam, A.ph.rhomin, A.ph.pgmin, out);
+    for (int it = 0; it < NIT; ++it) {
+        const int s = tid + 256 * it;
+        if (s < T::CELLS) {
+            const long long idx = cell_index(s);
 #pragma unroll
-        for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
+            for (int m = 0; m < 8; ++m) qs[it][m] = A.q[(long long)kPerm[DIR][m] * g.ncell + idx];
+        }
+    }
 #pragma unroll
-        for (int f = 0; f < O_COUNT; ++f) lo[f][s] = out[C_VN + f];
-        if constexpr (MODE != SW_PLAIN)
-            lbmag[s] = sqrtf((float)(out[C_BN] * out[C_BN] + out[C_BT1] * out[C_BT1] + out[C_BT2] * out[C_BT2]));
+    for (int it = 0; it < NIT; ++it) {
+        const int s = tid + 256 * it;
+        if (s < T::CELLS) {
+            R out[C_NFIELD];
+            cell_values<R>(qs[it], A.ph.gam, A.ph.rhomin, A.ph.pgmin, out);
+#pragma unroll
+            for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
+#pragma unroll
+            for (int f = 0; f < O_COUNT; ++f) lo[f][s] = out[C_VN + f];
+            if constexpr (MODE != SW_PLAIN)
+                lbmag[s] = sqrtf((float)(out[C_BN] * out[C_BN] + out[C_BT1] * out[C_BT1] + out[C_BT2] * out[C_BT2]));
+        }
     }
     __syncthreads();
 
@@ -320,11 +339,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
         if (A.stats) {
             const unsigned long long fb = __ballot(flagged), vb = __ballot(valid);
             if ((tid & 63) == 0 && vb) {
-                unsigned long long* st = A.stats + 4 * ((blockIdx.x + blockIdx.y + blockIdx.z) & 63);
-                atomicAdd(&st[0], (unsigned long long)__popcll(fb));
-                atomicAdd(&st[1], (unsigned long long)__popcll(vb));
-                atomicAdd(&st[2], fb ? 1ull : 0ull);
-                atomicAdd(&st[3], 1ull);
+                const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+                uint4* st = A.stats + 4 * blk + (tid >> 6);
+                uint4 v = *st;
+                v.x += (unsigned)__popcll(fb);
+                v.y += (unsigned)__popcll(vb);
+                v.z += fb ? 1u : 0u;
+                v.w += 1u;
+                *st = v;
             }
         }
     }

@@ -34,18 +34,10 @@ inline RK rk_coeff(int iw) {
 
 template <typename R, int MODE>
 void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
-    const Geom& g = c->g;
-    const int nif = g.per[d] ? g.n[d] : g.n[d] + 1;
-    if (d == 0) {
-        dim3 grid(blocks_for(nif, SweepTile<0>::LANES), blocks_for(g.n[1], SweepTile<0>::ROWS), g.n[2]);
-        hipLaunchKernelGGL((k_sweep<R, 0, MODE, EX>), grid, dim3(256), 0, c->stream, a);
-    } else if (d == 1) {
-        dim3 grid(blocks_for(g.n[0], SweepTile<1>::LANES), blocks_for(nif, SweepTile<1>::ROWS), g.n[2]);
-        hipLaunchKernelGGL((k_sweep<R, 1, MODE, EX>), grid, dim3(256), 0, c->stream, a);
-    } else {
-        dim3 grid(blocks_for(g.n[0], SweepTile<2>::LANES), blocks_for(nif, SweepTile<2>::ROWS), g.n[1]);
-        hipLaunchKernelGGL((k_sweep<R, 2, MODE, EX>), grid, dim3(256), 0, c->stream, a);
-    }
+    const dim3 grid = sweep_grid(c->g, d);
+    if (d == 0) hipLaunchKernelGGL((k_sweep<R, 0, MODE, EX>), grid, dim3(256), 0, c->stream, a);
+    else if (d == 1) hipLaunchKernelGGL((k_sweep<R, 1, MODE, EX>), grid, dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_sweep<R, 2, MODE, EX>), grid, dim3(256), 0, c->stream, a);
 }
 
 template <typename R>
@@ -60,7 +52,7 @@ SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     a.g = c->g;
     a.ph = c->ph;
     a.mlp = c->mlp;
-    a.stats = c->route_now ? c->istats : nullptr;
+    a.stats = c->route_now ? c->iwave : nullptr;
     return a;
 }
 

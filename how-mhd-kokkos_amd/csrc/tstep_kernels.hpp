@@ -57,6 +57,33 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+// Adds the per-wavefront counters of the routed sweeps (n slots of
+// (flagged, total, wavefronts flagged, wavefronts total)) to the four totals
+// and clears the slots.  Grid-stride loop over the slots; one atomic per
+// total and block.
+__global__ __launch_bounds__(256) void k_fold_wave_stats(uint4* w, long long n, unsigned long long* tot) {
+    unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const uint4 v = w[i];
+        s0 += v.x; s1 += v.y; s2 += v.z; s3 += v.w;
+        w[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_down(s0, o, 64);
+        s1 += __shfl_down(s1, o, 64);
+        s2 += __shfl_down(s2, o, 64);
+        s3 += __shfl_down(s3, o, 64);
+    }
+    __shared__ unsigned long long red[4][4];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wv] = s0; red[1][wv] = s1; red[2][wv] = s2; red[3][wv] = s3; }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        atomicAdd(&tot[k], red[k][0] + red[k][1] + red[k][2] + red[k][3]);
+    }
+}
+
 // initialize_bfield_from_q, src/fluxct.cpp:65-107, on active faces
 __global__ __launch_bounds__(256) void k_init_faces(const double* q, double* b, Geom g) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
