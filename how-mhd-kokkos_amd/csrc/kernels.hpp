@@ -210,7 +210,7 @@ struct Seven {
 #define HOWMHD_ROUTED_WAVES 4
 #endif
 template <typename R, int MODE>
-constexpr int sweep_waves() { return (sizeof(R) == 4 && MODE == SW_ROUTED) ? HOWMHD_ROUTED_WAVES : HOWMHD_SWEEP_WAVES; }
+constexpr int sweep_waves() { return (sizeof(R) == 4 && MODE != SW_COLLECT) ? HOWMHD_ROUTED_WAVES : HOWMHD_SWEEP_WAVES; }
 
 template <typename R, int DIR, int MODE, int EX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, MODE>()))) void k_sweep(

@@ -51,6 +51,13 @@ __device__ __forceinline__ R div_by(R a, R b, R y) {
     return (r == R(0)) ? q : __builtin_fma(r, y, q);
 }
 
+// 1/x in the FMA forms: FP32 takes the hardware reciprocal (v_rcp_f32,
+// within 1 ulp) instead of the IEEE division sequence; FP64 divides.
+template <typename R>
+__device__ __forceinline__ R rcp_fast(R x) { return R(1) / x; }
+template <>
+__device__ __forceinline__ float rcp_fast<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // x / c for a compile-time constant c with inv_c = RN(1/c).  Reference
 // rounding: the correctly rounded quotient (div_by).  FMA form:
 // multiplication by the rounded reciprocal (differs by at most one rounding).
@@ -165,7 +172,7 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
         const R lf2 = R(0.5) * (bbn2 + cs2 + root);
         lf = rsqrt_(lf2);
         la = rsqrt_(bnx2);
-        ls = (lf2 > R(0)) ? rsqrt_(bnx2 * cs2 / lf2) : R(0);
+        ls = (lf2 > R(0)) ? rsqrt_(bnx2 * cs2 * rcp_fast(lf2)) : R(0);
     }
 #endif
 
@@ -209,12 +216,12 @@ __device__ __forceinline__ void eig_scalars_fp32_speeds(Eig<R>& e, R cs2, R bn2,
     const R root = rsqrt_(dd * dd + R(4) * bt2 * c2);
     const R lf2 = R(0.5) * (b2 + c2 + root);
     e.lf = rsqrt_(lf2);
-    e.ls = (lf2 > R(0)) ? rsqrt_(bn2 * c2 / lf2) : R(0);
+    e.ls = (lf2 > R(0)) ? rsqrt_(bn2 * c2 * rcp_fast(lf2)) : R(0);
     if (root >= R(1.0e-30)) {
         const R q = R(4) * bt2 * c2;
-        const R fnum = (dd <= R(0)) ? root - dd : q / (root + dd);   // cs^2 - b^2 + root
-        const R snum = (dd >= R(0)) ? root + dd : q / (root - dd);   // b^2 - cs^2 + root
-        const R inv = R(0.5) / root;
+        const R fnum = (dd <= R(0)) ? root - dd : q * rcp_fast(root + dd);   // cs^2 - b^2 + root
+        const R snum = (dd >= R(0)) ? root + dd : q * rcp_fast(root - dd);   // b^2 - cs^2 + root
+        const R inv = R(0.5) * rcp_fast(root);
         e.af = rsqrt_(rmax(R(0), fnum * inv));
         e.as = rsqrt_(rmax(R(0), snum * inv));
     } else {
@@ -240,7 +247,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     const R bbn2 = BB2 / rho;
     const R bnx2 = Bx * Bx / rho;
 #else
-    const R irho = R(1) / rho;
+    const R irho = rcp_fast(rho);
     const R bbn2 = BB2 * irho;
     const R bnx2 = Bx * Bx * irho;
 #endif
@@ -263,7 +270,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
         af = rsqrt_(rmax(R(0), cs2 - ls * ls)) / rsqrt_(lf * lf - ls * ls);
         as = rsqrt_(rmax(R(0), lf * lf - cs2)) / rsqrt_(lf * lf - ls * ls);
 #else
-        const R rd = R(1) / rsqrt_(lf * lf - ls * ls);
+        const R rd = rcp_fast(rsqrt_(lf * lf - ls * ls));
         af = rsqrt_(rmax(R(0), cs2 - ls * ls)) * rd;
         as = rsqrt_(rmax(R(0), lf * lf - cs2)) * rd;
 #endif
@@ -280,7 +287,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
         bty = By / rsqrt_(Bt2);
         btz = Bz / rsqrt_(Bt2);
 #else
-        const R rb = R(1) / rsqrt_(Bt2);
+        const R rb = rcp_fast(rsqrt_(Bt2));
         bty = By * rb;
         btz = Bz * rb;
 #endif
@@ -293,11 +300,15 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     e.cs = cs; e.cs2 = cs2;
     e.bty = bty; e.btz = btz; e.sgnBx = sign1(Bx);
     e.sqrtrho = sqrtrho;
+#if HOWMHD_EXACT
     e.invsqrtrho = R(1) / sqrtrho;
+#else
+    e.invsqrtrho = rcp_fast(sqrtrho);
+#endif
     e.sgnBt = (By != R(0)) ? sign1(By) : sign1(Bz);
     e.slow_flip = (cs >= la);
 #if !HOWMHD_EXACT
-    const R ics2 = R(1) / cs2;
+    const R ics2 = rcp_fast(cs2);
     e.nrm_half = R(0.5) * ics2;
     e.nrm_ent = (gam - R(1)) * ics2;
 #else
@@ -340,7 +351,11 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
         L[5] = gam0 * as * Bz - cs * af * btz * sqrtrho;
         L[6] = -gam0 * as;
     } else if constexpr (M == 3) {
+#if HOWMHD_EXACT
         L[0] = -cs2 / gam0 - R(0.5) * vv2;
+#else
+        L[0] = -cs2 * rcp_fast(gam0) - R(0.5) * vv2;
+#endif
         L[1] = vx;
         L[2] = vy;
         L[3] = vz;
@@ -402,7 +417,11 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
 // Column m of R after the sign flip (src/ssprk.cpp:729-787, 808-818).
 template <int M, typename R>
 __device__ __forceinline__ void r_col(R gam, const Eig<R>& e, R* Rc) {
+#if HOWMHD_EXACT
     const R gam2 = (gam - R(2)) / (gam - R(1));
+#else
+    const R gam2 = (gam - R(2)) * rcp_fast(gam - R(1));
+#endif
     const R af = e.af, as = e.as, lf = e.lf, ls = e.ls, cs = e.cs, cs2 = e.cs2;
     const R vx = e.vx, vy = e.vy, vz = e.vz, vv2 = e.vv2;
     const R bty = e.bty, btz = e.btz, sgnBx = e.sgnBx, isr = e.invsqrtrho;
@@ -513,10 +532,10 @@ __device__ __forceinline__ R weno_side(R a, R b, R c, R d, R* w0_out = nullptr, 
         omega0 = t12 * inv;
         omega2 = R(3) * t01 * inv;
     } else {
-        const R alpha0 = R(1) / t0;
-        const R alpha1 = R(6) / t1;
-        const R alpha2 = R(3) / t2;
-        const R inv = R(1) / (alpha0 + alpha1 + alpha2);
+        const R alpha0 = rcp_fast(t0);
+        const R alpha1 = R(6) * rcp_fast(t1);
+        const R alpha2 = R(3) * rcp_fast(t2);
+        const R inv = rcp_fast(alpha0 + alpha1 + alpha2);
         omega0 = alpha0 * inv;
         omega2 = alpha2 * inv;
     }
