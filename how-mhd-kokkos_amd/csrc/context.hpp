@@ -43,6 +43,9 @@ struct howmhd_gpu_ctx {
     bool route_now = false;              // the running step routes
     uint4* iwave = nullptr;
     long long nwave = 0;
+    // reference-rounding sweeps: blocks to recompute with IEEE divisions
+    // (nwave / 4 flags, kept zero between sweeps)
+    unsigned* fixflag = nullptr;
     unsigned long long* istats = nullptr;
     // mode 1: counts of the last routed step travel to pinned host memory
     // behind an event; a step whose wavefronts were mostly flagged sends the

@@ -119,6 +119,11 @@ struct SweepArgs {
     // k_fold_wave_stats sums and clears after the step
     const MlpDev* mlp;
     uint4* stats;
+    // reference-rounding form: one flag per block (block-linear index), set
+    // by the sweep with unscaled WENO divisions where an operand left their
+    // range; the IEEE-division launch that follows recomputes those blocks
+    // and clears their flags
+    unsigned* fix;
 };
 
 // dF layout per direction: component-major, interface index t = 0..n along the
@@ -203,6 +208,106 @@ template <typename R>
 struct Seven {
     R v0, v1, v2, v3, v4, v5, v6;
 };
+// Characteristic LLF-WENO5 flux of the interface of phase-2 thread (row,
+// lane) from the sweep tile in LDS (src/ssprk.cpp:825-946): interface
+// eigensystem, the 7 wave maxima, then the waves in pairs, added to dFo in
+// wave order.  The WENO weights divide in mode DM (DivMode); `bad` collects
+// the lanes whose operands left the range of the unscaled division.
+template <typename R, int DIR, int MODE, int DM>
+__device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>::CELLS],
+                                               const R (*lo)[SweepTile<DIR>::CELLS], int row, int lane, R gam,
+                                               R (&dFo)[7], R& mdev, unsigned long long& bad) {
+    using T = SweepTile<DIR>;
+    (void)mdev;
+    auto slot = [&](int k) -> int {
+        if constexpr (DIR == 0) return row * (T::LANES + 5) + lane + k;
+        else return (row + k) * T::LANES + lane;
+    };
+    // interface state from cells k = 2, 3: (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H)
+    R u0[9], u1[9];
+    {
+        const int s0 = slot(2), s1 = slot(3);
+        u0[0] = luf[0][s0].u; u0[1] = lo[O_VN][s0]; u0[2] = lo[O_VT1][s0]; u0[3] = lo[O_VT2][s0];
+        u0[4] = lo[O_BN][s0]; u0[5] = luf[4][s0].u; u0[6] = luf[5][s0].u; u0[7] = lo[O_P][s0]; u0[8] = lo[O_H][s0];
+        u1[0] = luf[0][s1].u; u1[1] = lo[O_VN][s1]; u1[2] = lo[O_VT1][s1]; u1[3] = lo[O_VT2][s1];
+        u1[4] = lo[O_BN][s1]; u1[5] = luf[4][s1].u; u1[6] = luf[5][s1].u; u1[7] = lo[O_P][s1]; u1[8] = lo[O_H][s1];
+    }
+        Eig<R> e;
+    eig_scalars<R>(gam, u0, u1, e);
+
+    // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
+    Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int s = slot(k);
+        const R vn = lo[O_VN][s], lf = lo[O_LF][s], la = lo[O_LA][s], ls = lo[O_LS][s];
+        amx.v0 = rmax(amx.v0, rabs(vn - lf));
+        amx.v1 = rmax(amx.v1, rabs(vn - la));
+        amx.v2 = rmax(amx.v2, rabs(vn - ls));
+        amx.v3 = rmax(amx.v3, rabs(vn));
+        amx.v4 = rmax(amx.v4, rabs(vn + ls));
+        amx.v5 = rmax(amx.v5, rabs(vn + la));
+        amx.v6 = rmax(amx.v6, rabs(vn + lf));
+    }
+
+    // Waves are taken in pairs (0,1), (2,3), (4,5), (6): both waves of a pair
+    // are projected in one pass over the stencil in LDS, and dF still gets
+    // the waves added in order 0..6.
+    const R amxv[7] = {amx.v0, amx.v1, amx.v2, amx.v3, amx.v4, amx.v5, amx.v6};
+    sfor<0, 4>([&](auto pc) {
+        constexpr int m0 = 2 * decltype(pc)::value;
+        constexpr bool two = (m0 + 1 < 7);
+        constexpr int m1 = two ? m0 + 1 : m0;
+        // the stencil is read from LDS again for every pair rather than kept
+        // live in registers
+        asm volatile("" ::: "memory");
+        // the scalars become opaque here, so the compiler builds this pair's
+        // rows and columns here instead of all seven up front (98 live values)
+        opaque(e);
+        R La[7], Lb[7];
+        l_row<m0>(gam, e, La);
+        if constexpr (two) l_row<m1>(gam, e, Lb);
+        R Fa[6], qa[6], Fb[6], qb[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int s = slot(k);
+            R U[7], F[7];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const UF<R> v = luf[j][s];
+                U[j] = v.u;
+                F[j] = v.f;
+            }
+            Fa[k] = La[0] * F[0] + La[1] * F[1] + La[2] * F[2] + La[3] * F[3] + La[4] * F[4] + La[5] * F[5] + La[6] * F[6];
+            qa[k] = La[0] * U[0] + La[1] * U[1] + La[2] * U[2] + La[3] * U[3] + La[4] * U[4] + La[5] * U[5] + La[6] * U[6];
+            if constexpr (two) {
+                Fb[k] = Lb[0] * F[0] + Lb[1] * F[1] + Lb[2] * F[2] + Lb[3] * F[3] + Lb[4] * F[4] + Lb[5] * F[5] + Lb[6] * F[6];
+                qb[k] = Lb[0] * U[0] + Lb[1] * U[1] + Lb[2] * U[2] + Lb[3] * U[3] + Lb[4] * U[4] + Lb[5] * U[5] + Lb[6] * U[6];
+            }
+            if ((k + 1) % HOWMHD_PROJ_BATCH == 0) asm volatile("" ::: "memory");
+        }
+        R Fsa, Fsb = R(0);
+        if constexpr (MODE == SW_COLLECT) {
+            Fsa = weno_wave_dev<R, DM>(Fa, qa, amxv[m0], mdev, bad);
+            if constexpr (two) Fsb = weno_wave_dev<R, DM>(Fb, qb, amxv[m1], mdev, bad);
+        } else {
+            Fsa = weno_wave<R, DM>(Fa, qa, amxv[m0], bad);
+            if constexpr (two) Fsb = weno_wave<R, DM>(Fb, qb, amxv[m1], bad);
+        }
+        opaque_after(e, Fsa);
+        if constexpr (two) opaque_after(e, Fsb);
+        R Rc[7];
+        r_col<m0>(gam, e, Rc);
+#pragma unroll
+        for (int r = 0; r < 7; ++r) dFo[r] += Rc[r] * Fsa;
+        if constexpr (two) {
+            r_col<m1>(gam, e, Rc);
+#pragma unroll
+            for (int r = 0; r < 7; ++r) dFo[r] += Rc[r] * Fsb;
+        }
+    });
+}
+
 // Wavefronts per SIMD the sweep is compiled for: FP32 routed sweeps 4 (at
 // most 128 registers; four blocks' tiles fill the 160 KB of LDS), the others
 // HOWMHD_SWEEP_WAVES.
@@ -212,11 +317,22 @@ struct Seven {
 template <typename R, int MODE>
 constexpr int sweep_waves() { return (sizeof(R) == 4 && MODE != SW_COLLECT) ? HOWMHD_ROUTED_WAVES : HOWMHD_SWEEP_WAVES; }
 
-template <typename R, int DIR, int MODE, int EX>
+// DM (DivMode) of the WENO weights: the FMA forms and the reference-rounding
+// recompute launch divide with IEEE divisions; the reference-rounding sweep
+// proper with div_recip/div_with, flagging its block in A.fix where an
+// operand leaves their range.  A DIV_IEEE launch of the reference-rounding
+// form only runs the blocks whose flag is set.
+template <typename R, int DIR, int MODE, int EX, int DM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, MODE>()))) void k_sweep(
     SweepArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
+    static_assert(EX || DM == DIV_IEEE, "the FMA forms divide with IEEE divisions");
     using T = SweepTile<DIR>;
+    constexpr bool RECOMPUTE = EX && DM == DIV_IEEE;
+    const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    if constexpr (RECOMPUTE) {
+        if (A.fix[blk] == 0u) return;  // the same for the whole block
+    }
     __shared__ UF<R> luf[7][T::CELLS];
     __shared__ R lo[O_COUNT][T::CELLS];
     // |B| of each cell for the indicator features (routed and sample sweeps)
@@ -291,6 +407,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
         }
     }
     __syncthreads();
+    if constexpr (RECOMPUTE) {
+        if (threadIdx.x == 0) A.fix[blk] = 0u;  // every thread has read it
+    }
 
     // ---- phase 2: one interface per thread
     int lane, row, t, x, y, z;
@@ -339,7 +458,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
         if (A.stats) {
             const unsigned long long fb = __ballot(flagged), vb = __ballot(valid);
             if ((tid & 63) == 0 && vb) {
-                const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
                 uint4* st = A.stats + 4 * blk + (tid >> 6);
                 uint4 v = *st;
                 v.x += (unsigned)__popcll(fb);
@@ -355,90 +473,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
     R dF[7] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
     R maxdev = R(0);
     if (MODE != SW_ROUTED || flagged) {
-    // interface state from cells k = 2, 3: (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H)
-    R u0[9], u1[9];
-    {
-        const int s0 = slot(2), s1 = slot(3);
-        u0[0] = luf[0][s0].u; u0[1] = lo[O_VN][s0]; u0[2] = lo[O_VT1][s0]; u0[3] = lo[O_VT2][s0];
-        u0[4] = lo[O_BN][s0]; u0[5] = luf[4][s0].u; u0[6] = luf[5][s0].u; u0[7] = lo[O_P][s0]; u0[8] = lo[O_H][s0];
-        u1[0] = luf[0][s1].u; u1[1] = lo[O_VN][s1]; u1[2] = lo[O_VT1][s1]; u1[3] = lo[O_VT2][s1];
-        u1[4] = lo[O_BN][s1]; u1[5] = luf[4][s1].u; u1[6] = luf[5][s1].u; u1[7] = lo[O_P][s1]; u1[8] = lo[O_H][s1];
-    }
-    const R gam = R(A.ph.gam);
-    Eig<R> e;
-    eig_scalars<R>(gam, u0, u1, e);
-
-    // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
-    Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const int s = slot(k);
-        const R vn = lo[O_VN][s], lf = lo[O_LF][s], la = lo[O_LA][s], ls = lo[O_LS][s];
-        amx.v0 = rmax(amx.v0, rabs(vn - lf));
-        amx.v1 = rmax(amx.v1, rabs(vn - la));
-        amx.v2 = rmax(amx.v2, rabs(vn - ls));
-        amx.v3 = rmax(amx.v3, rabs(vn));
-        amx.v4 = rmax(amx.v4, rabs(vn + ls));
-        amx.v5 = rmax(amx.v5, rabs(vn + la));
-        amx.v6 = rmax(amx.v6, rabs(vn + lf));
-    }
-
-    // Waves are taken in pairs (0,1), (2,3), (4,5), (6): both waves of a pair
-    // are projected in one pass over the stencil in LDS, and dF still gets
-    // the waves added in order 0..6.
-    const R amxv[7] = {amx.v0, amx.v1, amx.v2, amx.v3, amx.v4, amx.v5, amx.v6};
-    sfor<0, 4>([&](auto pc) {
-        constexpr int m0 = 2 * decltype(pc)::value;
-        constexpr bool two = (m0 + 1 < 7);
-        constexpr int m1 = two ? m0 + 1 : m0;
-        // the stencil is read from LDS again for every pair rather than kept
-        // live in registers
-        asm volatile("" ::: "memory");
-        // the scalars become opaque here, so the compiler builds this pair's
-        // rows and columns here instead of all seven up front (98 live values)
-        opaque(e);
-        R La[7], Lb[7];
-        l_row<m0>(gam, e, La);
-        if constexpr (two) l_row<m1>(gam, e, Lb);
-        R Fa[6], qa[6], Fb[6], qb[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int s = slot(k);
-            R U[7], F[7];
-#pragma unroll
-            for (int j = 0; j < 7; ++j) {
-                const UF<R> v = luf[j][s];
-                U[j] = v.u;
-                F[j] = v.f;
-            }
-            Fa[k] = La[0] * F[0] + La[1] * F[1] + La[2] * F[2] + La[3] * F[3] + La[4] * F[4] + La[5] * F[5] + La[6] * F[6];
-            qa[k] = La[0] * U[0] + La[1] * U[1] + La[2] * U[2] + La[3] * U[3] + La[4] * U[4] + La[5] * U[5] + La[6] * U[6];
-            if constexpr (two) {
-                Fb[k] = Lb[0] * F[0] + Lb[1] * F[1] + Lb[2] * F[2] + Lb[3] * F[3] + Lb[4] * F[4] + Lb[5] * F[5] + Lb[6] * F[6];
-                qb[k] = Lb[0] * U[0] + Lb[1] * U[1] + Lb[2] * U[2] + Lb[3] * U[3] + Lb[4] * U[4] + Lb[5] * U[5] + Lb[6] * U[6];
-            }
-            if ((k + 1) % HOWMHD_PROJ_BATCH == 0) asm volatile("" ::: "memory");
+        unsigned long long bad = 0ull;
+        weno_interface<R, DIR, MODE, DM>(luf, lo, row, lane, R(A.ph.gam), dF, maxdev, bad);
+        // a WENO weight operand beyond the unscaled-division range: the block
+        // is recomputed with IEEE divisions by the launch that follows
+        if constexpr (EX && DM == DIV_UNSCALED) {
+            if (bad) A.fix[blk] = 1u;
         }
-        R Fsa, Fsb = R(0);
-        if constexpr (MODE == SW_COLLECT) {
-            Fsa = weno_wave_dev<R>(Fa, qa, amxv[m0], maxdev);
-            if constexpr (two) Fsb = weno_wave_dev<R>(Fb, qb, amxv[m1], maxdev);
-        } else {
-            Fsa = weno_wave<R>(Fa, qa, amxv[m0]);
-            if constexpr (two) Fsb = weno_wave<R>(Fb, qb, amxv[m1]);
-        }
-        opaque_after(e, Fsa);
-        if constexpr (two) opaque_after(e, Fsb);
-        R Rc[7];
-        r_col<m0>(gam, e, Rc);
-#pragma unroll
-        for (int r = 0; r < 7; ++r) dF[r] += Rc[r] * Fsa;
-        if constexpr (two) {
-            r_col<m1>(gam, e, Rc);
-#pragma unroll
-            for (int r = 0; r < 7; ++r) dF[r] += Rc[r] * Fsb;
-        }
-    });
     } else {
         // linear upwind-5 with one LLF speed: the largest |v_n| + c_f over the
         // stencil bounds every wave's speed

@@ -51,6 +51,36 @@ __device__ __forceinline__ R div_by(R a, R b, R y) {
     return (r == R(0)) ? q : __builtin_fma(r, y, q);
 }
 
+// IEEE division a / b in parts, for operands whose division needs no range
+// scaling.  The correctly rounded FP64 division sequence of gfx9 (v_div_scale,
+// v_rcp_f64, two Newton steps, q = a r, remainder fma(-b, q, a), v_div_fmas,
+// v_div_fixup) leaves its operands unscaled and the quotient untouched when
+// b and a/b are normal, 1/b is normal, |a| >= 2^-969 and exponent(a) -
+// exponent(b) < 768; the arithmetic is then div_recip (the refined
+// reciprocal, a function of b alone) and div_with.  So quotients that share a
+// denominator share its refinement and stay bit-for-bit the IEEE quotient.
+__device__ __forceinline__ double div_recip(double b) {
+    const double r0 = __builtin_amdgcn_rcp(b);
+    const double r1 = __builtin_fma(r0, __builtin_fma(-b, r0, 1.0), r0);
+    return __builtin_fma(r1, __builtin_fma(-b, r1, 1.0), r1);
+}
+__device__ __forceinline__ double div_with(double a, double b, double r) {
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+}
+
+// Division form of the reference-rounding WENO weights: IEEE divisions, or
+// div_recip/div_with, which hold for t_k = (eps + IS_k)^2 <= 1e200 (then
+// alpha_k in [1e-200, 6e20] and omega in (5e-222, 1]).  The unscaled form
+// collects in `bad` (a wavefront mask, kept in scalar registers) the lanes
+// where the high word of a t_k bit pattern reaches kUnscaledHi, the high word
+// of 1e200 (every double with a smaller high word is below 1e200; t_k > 0,
+// and inf and NaN compare above every finite value); the caller recomputes
+// those with IEEE divisions.
+enum DivMode { DIV_IEEE = 0, DIV_UNSCALED = 1 };
+constexpr uint32_t kUnscaledHi = 0x6974E718u;
+__device__ __forceinline__ uint32_t hi_word(double x) { return (uint32_t)((unsigned long long)__double_as_longlong(x) >> 32); }
+
 // 1/x in the FMA forms: FP32 takes the hardware reciprocal (v_rcp_f32,
 // within 1 ulp) instead of the IEEE division sequence; FP64 divides.
 template <typename R>
@@ -500,18 +530,33 @@ __device__ __forceinline__ void r_col(R gam, const Eig<R>& e, R* Rc) {
 // phi_N(a,b,c,d) = w0 (a-2b+c)/3 + (w2-1/2)(b-2c+d)/6 with
 // IS0 = 13(a-b)^2 + 3(a-3b)^2, IS1 = 13(b-c)^2 + 3(b+c)^2,
 // IS2 = 13(c-d)^2 + 3(3c-d)^2, alpha = (1,6,3)/(eps+IS)^2, eps = 1e-10.
-template <typename R>
-__device__ __forceinline__ R weno_side(R a, R b, R c, R d, R* w0_out = nullptr, R* w2_out = nullptr) {
+template <typename R, int DM = DIV_IEEE>
+__device__ __forceinline__ R weno_side(R a, R b, R c, R d, unsigned long long& bad, R* w0_out = nullptr, R* w2_out = nullptr) {
     const R eps = R(1.0e-10);
     const R IS0 = R(13) * (a - b) * (a - b) + R(3) * (a - R(3) * b) * (a - R(3) * b);
     const R IS1 = R(13) * (b - c) * (b - c) + R(3) * (b + c) * (b + c);
     const R IS2 = R(13) * (c - d) * (c - d) + R(3) * (R(3) * c - d) * (R(3) * c - d);
 #if HOWMHD_EXACT
-    const R alpha0 = R(1) / ((eps + IS0) * (eps + IS0));
-    const R alpha1 = R(6) / ((eps + IS1) * (eps + IS1));
-    const R alpha2 = R(3) / ((eps + IS2) * (eps + IS2));
-    const R omega0 = alpha0 / (alpha0 + alpha1 + alpha2);
-    const R omega2 = alpha2 / (alpha0 + alpha1 + alpha2);
+    const R t0 = (eps + IS0) * (eps + IS0);
+    const R t1 = (eps + IS1) * (eps + IS1);
+    const R t2 = (eps + IS2) * (eps + IS2);
+    R omega0, omega2;
+    if constexpr (DM == DIV_UNSCALED) {
+        bad |= __ballot(max(hi_word(t0), max(hi_word(t1), hi_word(t2))) >= kUnscaledHi);
+        const R alpha0 = div_with(R(1), t0, div_recip(t0));
+        const R alpha1 = div_with(R(6), t1, div_recip(t1));
+        const R alpha2 = div_with(R(3), t2, div_recip(t2));
+        const R S = alpha0 + alpha1 + alpha2;
+        const R rS = div_recip(S);
+        omega0 = div_with(alpha0, S, rS);
+        omega2 = div_with(alpha2, S, rS);
+    } else {
+        const R alpha0 = R(1) / t0;
+        const R alpha1 = R(6) / t1;
+        const R alpha2 = R(3) / t2;
+        omega0 = alpha0 / (alpha0 + alpha1 + alpha2);
+        omega2 = alpha2 / (alpha0 + alpha1 + alpha2);
+    }
     if (w0_out) { *w0_out = omega0; *w2_out = omega2; }
     return div_c<R>(omega0 * (a - R(2) * b + c), R(3), R(1.0 / 3.0)) +
            div_c<R>((omega2 - R(0.5)) * (b - R(2) * c + d), R(6), R(1.0 / 6.0));
@@ -546,8 +591,8 @@ __device__ __forceinline__ R weno_side(R a, R b, R c, R d, R* w0_out = nullptr, 
 
 // Characteristic flux of one wave from its 6 projected values
 // (src/ssprk.cpp:883-937): first - phi(plus side) + phi(minus side).
-template <typename R>
-__device__ __forceinline__ R weno_wave(const R* Fsk, const R* qsk, R amx) {
+template <typename R, int DM = DIV_IEEE>
+__device__ __forceinline__ R weno_wave(const R* Fsk, const R* qsk, R amx, unsigned long long& bad) {
     R dF[5], dq[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -555,18 +600,18 @@ __device__ __forceinline__ R weno_wave(const R* Fsk, const R* qsk, R amx) {
         dq[k] = qsk[k + 1] - qsk[k];
     }
     const R first = div_c<R>(-Fsk[1] + R(7) * Fsk[2] + R(7) * Fsk[3] - Fsk[4], R(12), R(1.0 / 12.0));
-    const R second = weno_side<R>(R(0.5) * (dF[0] + amx * dq[0]), R(0.5) * (dF[1] + amx * dq[1]),
-                                  R(0.5) * (dF[2] + amx * dq[2]), R(0.5) * (dF[3] + amx * dq[3]));
-    const R third = weno_side<R>(R(0.5) * (dF[4] - amx * dq[4]), R(0.5) * (dF[3] - amx * dq[3]),
-                                 R(0.5) * (dF[2] - amx * dq[2]), R(0.5) * (dF[1] - amx * dq[1]));
+    const R second = weno_side<R, DM>(R(0.5) * (dF[0] + amx * dq[0]), R(0.5) * (dF[1] + amx * dq[1]),
+                                      R(0.5) * (dF[2] + amx * dq[2]), R(0.5) * (dF[3] + amx * dq[3]), bad);
+    const R third = weno_side<R, DM>(R(0.5) * (dF[4] - amx * dq[4]), R(0.5) * (dF[3] - amx * dq[3]),
+                                     R(0.5) * (dF[2] - amx * dq[2]), R(0.5) * (dF[1] - amx * dq[1]), bad);
     return first - second + third;
 }
 
 // weno_wave that also raises `dev` to the largest |w_r - d_r| of both sides,
 // d = (0.1, 0.6, 0.3) (the label of the learned indicator; weights of
 // src/ssprk.cpp:903-908, 926-931, w1 = 1 - w0 - w2).
-template <typename R>
-__device__ __forceinline__ R weno_wave_dev(const R* Fsk, const R* qsk, R amx, R& dev) {
+template <typename R, int DM = DIV_IEEE>
+__device__ __forceinline__ R weno_wave_dev(const R* Fsk, const R* qsk, R amx, R& dev, unsigned long long& bad) {
     R dF[5], dq[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -575,11 +620,11 @@ __device__ __forceinline__ R weno_wave_dev(const R* Fsk, const R* qsk, R amx, R&
     }
     const R first = div_c<R>(-Fsk[1] + R(7) * Fsk[2] + R(7) * Fsk[3] - Fsk[4], R(12), R(1.0 / 12.0));
     R w0, w2;
-    const R second = weno_side<R>(R(0.5) * (dF[0] + amx * dq[0]), R(0.5) * (dF[1] + amx * dq[1]),
-                                  R(0.5) * (dF[2] + amx * dq[2]), R(0.5) * (dF[3] + amx * dq[3]), &w0, &w2);
+    const R second = weno_side<R, DM>(R(0.5) * (dF[0] + amx * dq[0]), R(0.5) * (dF[1] + amx * dq[1]),
+                                      R(0.5) * (dF[2] + amx * dq[2]), R(0.5) * (dF[3] + amx * dq[3]), bad, &w0, &w2);
     dev = rmax(dev, rmax(rabs(w0 - R(0.1)), rmax(rabs(w2 - R(0.3)), rabs(R(1) - w0 - w2 - R(0.6)))));
-    const R third = weno_side<R>(R(0.5) * (dF[4] - amx * dq[4]), R(0.5) * (dF[3] - amx * dq[3]),
-                                 R(0.5) * (dF[2] - amx * dq[2]), R(0.5) * (dF[1] - amx * dq[1]), &w0, &w2);
+    const R third = weno_side<R, DM>(R(0.5) * (dF[4] - amx * dq[4]), R(0.5) * (dF[3] - amx * dq[3]),
+                                     R(0.5) * (dF[2] - amx * dq[2]), R(0.5) * (dF[1] - amx * dq[1]), bad, &w0, &w2);
     dev = rmax(dev, rmax(rabs(w0 - R(0.1)), rmax(rabs(w2 - R(0.3)), rabs(R(1) - w0 - w2 - R(0.6)))));
     return first - second + third;
 }

@@ -32,12 +32,27 @@ inline RK rk_coeff(int iw) {
     return c;
 }
 
+template <typename R, int MODE, int DM>
+void launch_sweep_dm(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
+    const dim3 grid = sweep_grid(c->g, d);
+    if (d == 0) hipLaunchKernelGGL((k_sweep<R, 0, MODE, EX, DM>), grid, dim3(256), 0, c->stream, a);
+    else if (d == 1) hipLaunchKernelGGL((k_sweep<R, 1, MODE, EX, DM>), grid, dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_sweep<R, 2, MODE, EX, DM>), grid, dim3(256), 0, c->stream, a);
+}
+
+// The sweep along d.  Reference rounding: the sweep with unscaled WENO
+// divisions, then the IEEE-division launch for the blocks it flagged (most
+// blocks exit at once; no counting there).
 template <typename R, int MODE>
 void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
-    const dim3 grid = sweep_grid(c->g, d);
-    if (d == 0) hipLaunchKernelGGL((k_sweep<R, 0, MODE, EX>), grid, dim3(256), 0, c->stream, a);
-    else if (d == 1) hipLaunchKernelGGL((k_sweep<R, 1, MODE, EX>), grid, dim3(256), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_sweep<R, 2, MODE, EX>), grid, dim3(256), 0, c->stream, a);
+    if constexpr (EX) {
+        launch_sweep_dm<R, MODE, DIV_UNSCALED>(c, d, a);
+        SweepArgs<R> b = a;
+        b.stats = nullptr;
+        launch_sweep_dm<R, MODE, DIV_IEEE>(c, d, b);
+    } else {
+        launch_sweep_dm<R, MODE, DIV_IEEE>(c, d, a);
+    }
 }
 
 template <typename R>
@@ -53,6 +68,7 @@ SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     a.ph = c->ph;
     a.mlp = c->mlp;
     a.stats = c->route_now ? c->iwave : nullptr;
+    a.fix = c->fixflag;
     return a;
 }
 

@@ -503,6 +503,17 @@ static void line_boundary(line_ws* w, double gam) {
     }
 }
 
+/* Test hook: the number of WENO weight evaluations (one side of one wave)
+ * with some t_k = (eps + IS_k)^2 above 1e200 or not finite.  It lets a test
+ * show that its data reaches weight operands of that size. */
+static long long g_large_t = 0;
+static void note_large_t(double t0, double t1, double t2) {
+    if (!(t0 <= 1e200 && t1 <= 1e200 && t2 <= 1e200)) {
+#pragma omp atomic
+        g_large_t++;
+    }
+}
+
 /* Characteristic LLF-WENO5 flux of one wave, src/ssprk.cpp:883-937 */
 static double weno_wave(const double* Fsk, const double* qsk, double amx, double* wp, double* wm) {
     const double eps = 1.0e-10;
@@ -520,6 +531,7 @@ static double weno_wave(const double* Fsk, const double* qsk, double amx, double
     double IS0 = 13.0 * (aterm - bterm) * (aterm - bterm) + 3.0 * (aterm - 3.0 * bterm) * (aterm - 3.0 * bterm);
     double IS1 = 13.0 * (bterm - cterm) * (bterm - cterm) + 3.0 * (bterm + cterm) * (bterm + cterm);
     double IS2 = 13.0 * (cterm - dterm) * (cterm - dterm) + 3.0 * (3.0 * cterm - dterm) * (3.0 * cterm - dterm);
+    note_large_t((eps + IS0) * (eps + IS0), (eps + IS1) * (eps + IS1), (eps + IS2) * (eps + IS2));
     double alpha0 = 1.0 / ((eps + IS0) * (eps + IS0));
     double alpha1 = 6.0 / ((eps + IS1) * (eps + IS1));
     double alpha2 = 3.0 / ((eps + IS2) * (eps + IS2));
@@ -536,6 +548,7 @@ static double weno_wave(const double* Fsk, const double* qsk, double amx, double
     IS0 = 13.0 * (aterm - bterm) * (aterm - bterm) + 3.0 * (aterm - 3.0 * bterm) * (aterm - 3.0 * bterm);
     IS1 = 13.0 * (bterm - cterm) * (bterm - cterm) + 3.0 * (bterm + cterm) * (bterm + cterm);
     IS2 = 13.0 * (cterm - dterm) * (cterm - dterm) + 3.0 * (3.0 * cterm - dterm) * (3.0 * cterm - dterm);
+    note_large_t((eps + IS0) * (eps + IS0), (eps + IS1) * (eps + IS1), (eps + IS2) * (eps + IS2));
     alpha0 = 1.0 / ((eps + IS0) * (eps + IS0));
     alpha1 = 6.0 / ((eps + IS1) * (eps + IS1));
     alpha2 = 3.0 / ((eps + IS2) * (eps + IS2));
@@ -1083,6 +1096,12 @@ double howmhd_ref_weno_wave_weights(const double* Fsk, const double* qsk, double
 }
 
 double howmhd_ref_upwind5(const double* F, const double* U, double alpha) { return upwind5(F, U, alpha); }
+
+long long howmhd_ref_large_t_count(int reset) {
+    const long long n = g_large_t;
+    if (reset) g_large_t = 0;
+    return n;
+}
 
 int howmhd_ref_set_route(howmhd_ref_ctx* c, int route) {
     if (!c || route < 0 || route > 1) return HOWMHD_EINVAL;

@@ -72,6 +72,9 @@ double howmhd_ref_upwind5(const double* F, const double* U, double alpha);
 /* 0: characteristic WENO5 at every interface (default); 1: linear upwind-5
  * at every interface (the indicator form with no interface flagged). */
 int howmhd_ref_set_route(howmhd_ref_ctx* c, int route);
+/* Test hook: WENO weight evaluations so far with some t_k = (eps + IS_k)^2
+ * above 1e200 or not finite (reset != 0: start again from zero). */
+long long howmhd_ref_large_t_count(int reset);
 
 /* Indicator label source for one line (qline as in howmhd_ref_line_sweep,
  * not modified): maxdev[i], i = 0..nn, the largest deviation of the WENO

@@ -47,6 +47,7 @@ def lib() -> C.CDLL:
             "howmhd_ref_weno_wave_weights": (C.c_double, [_DP, _DP, C.c_double, _DP, _DP]),
             "howmhd_ref_upwind5": (C.c_double, [_DP, _DP, C.c_double]),
             "howmhd_ref_set_route": (C.c_int, [C.c_void_p, C.c_int]),
+            "howmhd_ref_large_t_count": (C.c_longlong, [C.c_int]),
             "howmhd_ref_line_maxdev": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, _DP, _DP]),
         }
         for fn, (res, args) in table.items():
@@ -169,3 +170,9 @@ def line_maxdev(qline: np.ndarray, gam: float, rhomin=1e-12, pgmin=1e-12):
     out = np.zeros(nn + 1)
     assert lib().howmhd_ref_line_maxdev(gam, rhomin, pgmin, nn, _ptr(q), _ptr(out)) == 0
     return out
+
+
+def large_t_count(reset=False):
+    """WENO weight evaluations so far with some t_k = (eps + IS_k)^2 above
+    1e200 or not finite."""
+    return int(lib().howmhd_ref_large_t_count(int(reset)))

@@ -74,6 +74,40 @@ def test_fp64_bitwise(case, require_gpu):
     assert nb == 0, f"{nb} face values differ"
 
 
+@pytest.mark.parametrize("problem,kw", [
+    ("mhd_blast3d", dict(nx=16, ny=16, nz=16)),
+    ("orszag_tang", dict(nx=32, ny=32, nz=1, cour=0.8)),
+])
+def test_fp64_bitwise_large_weno_operands(problem, kw, require_gpu):
+    """The same flow scaled by 1e50 (rho, momenta, E x 1e50, B x 1e25; the
+    velocities and the dt sequence stay the same): some WENO weight operands
+    t_k = (eps + IS_k)^2 exceed 1e200, where the reference-rounding sweep
+    hands its block to the IEEE-division launch.  Still bit for bit."""
+    run, q0, p = _setup(problem, kw)
+    q0 = q0.copy()
+    q0[[0, 1, 2, 3, 7]] *= 1e50
+    q0[[4, 5, 6]] *= 1e25
+    print(f"[parity fp64 large operands] {problem} {kw}", flush=True)
+    ref = O.RefSolver(p)
+    gpu = H.GpuSolver(p)
+    for s in (ref, gpu):
+        s.upload_q0(q0)
+        s.init_faces()
+    O.large_t_count(reset=True)
+    for step in range(2):
+        dt = ref.tstep()
+        assert gpu.tstep() == dt, f"dt differs at step {step}"
+        ref.ssprk(dt)
+        gpu.ssprk(dt)
+    nlarge = O.large_t_count()
+    print("weight evaluations with t_k > 1e200:", nlarge)
+    assert nlarge > 0
+    qr, qg = ref.q0(), gpu.q0()
+    assert np.all(np.isfinite(H.active(qr)))
+    assert _bitwise_mismatch(qg, qr) == 0
+    assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
+
+
 SMOOTH = {"smooth16", "cpaw", "mhdblast_open"}
 
 
