@@ -43,9 +43,12 @@ struct howmhd_gpu_ctx {
     bool route_now = false;              // the running step routes
     uint4* iwave = nullptr;
     long long nwave = 0;
-    // reference-rounding sweeps: blocks to recompute with IEEE divisions
-    // (nwave / 4 flags, kept zero between sweeps)
+    // reference-rounding sweeps: tiles to recompute with IEEE divisions
+    // (nwave / 4 flags and queue entries, kept zero/empty between sweeps;
+    // queue length and blocks done)
     unsigned* fixflag = nullptr;
+    unsigned* fixlist = nullptr;
+    unsigned* fixctl = nullptr;
     unsigned long long* istats = nullptr;
     // mode 1: counts of the last routed step travel to pinned host memory
     // behind an event; a step whose wavefronts were mostly flagged sends the
@@ -79,15 +82,6 @@ struct howmhd_gpu_ctx {
 
 namespace howmhd {
 
-inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
-
-// Launch grid of the sweep along axis d (256-thread blocks; SweepTile).
-inline dim3 sweep_grid(const Geom& g, int d) {
-    const int nif = g.per[d] ? g.n[d] : g.n[d] + 1;
-    if (d == 0) return dim3(blocks_for(nif, SweepTile<0>::LANES), blocks_for(g.n[1], SweepTile<0>::ROWS), g.n[2]);
-    if (d == 1) return dim3(blocks_for(g.n[0], SweepTile<1>::LANES), blocks_for(nif, SweepTile<1>::ROWS), g.n[2]);
-    return dim3(blocks_for(g.n[0], SweepTile<2>::LANES), blocks_for(nif, SweepTile<2>::ROWS), g.n[1]);
-}
 
 inline int fail(howmhd_gpu_ctx* c, int code, const std::string& msg) {
     c->err = msg;

@@ -223,6 +223,8 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     }
     ok = ok && alloc((void**)&c->iwave, (size_t)c->nwave * sizeof(uint4));
     ok = ok && alloc((void**)&c->fixflag, (size_t)(c->nwave / 4) * sizeof(unsigned));
+    ok = ok && alloc((void**)&c->fixlist, (size_t)(c->nwave / 4) * sizeof(unsigned));
+    ok = ok && alloc((void**)&c->fixctl, 2 * sizeof(unsigned));
     ok = ok && alloc((void**)&c->istats, 4 * sizeof(unsigned long long));
     ok = ok && hipHostMalloc((void**)&c->istats_host, 4 * sizeof(unsigned long long), hipHostMallocDefault) ==
                    hipSuccess;
@@ -251,6 +253,8 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
     if (c->istats) (void)hipFree(c->istats);
     if (c->iwave) (void)hipFree(c->iwave);
     if (c->fixflag) (void)hipFree(c->fixflag);
+    if (c->fixlist) (void)hipFree(c->fixlist);
+    if (c->fixctl) (void)hipFree(c->fixctl);
     if (c->istats_host) (void)hipHostFree(c->istats_host);
     if (c->istats_ev) (void)hipEventDestroy(c->istats_ev);
     drain_timing(c);

@@ -93,6 +93,16 @@ template <> struct SweepTile<2> {
 #define HOWMHD_PROJ_BATCH 3
 #endif
 
+__host__ __device__ inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Launch grid of the sweep along axis d (256-thread blocks; SweepTile).
+__host__ __device__ inline dim3 sweep_grid(const Geom& g, int d) {
+    const int nif = g.per[d] ? g.n[d] : g.n[d] + 1;
+    if (d == 0) return dim3(blocks_for(nif, SweepTile<0>::LANES), blocks_for(g.n[1], SweepTile<0>::ROWS), g.n[2]);
+    if (d == 1) return dim3(blocks_for(g.n[0], SweepTile<1>::LANES), blocks_for(nif, SweepTile<1>::ROWS), g.n[2]);
+    return dim3(blocks_for(g.n[0], SweepTile<2>::LANES), blocks_for(nif, SweepTile<2>::ROWS), g.n[1]);
+}
+
 // Sweep variants: plain characteristic WENO5; the same plus indicator
 // samples (features and the WENO-weight label); routed by the learned
 // indicator (flagged interfaces: characteristic WENO5, others: linear
@@ -119,11 +129,13 @@ struct SweepArgs {
     // k_fold_wave_stats sums and clears after the step
     const MlpDev* mlp;
     uint4* stats;
-    // reference-rounding form: one flag per block (block-linear index), set
-    // by the sweep with unscaled WENO divisions where an operand left their
-    // range; the IEEE-division launch that follows recomputes those blocks
-    // and clears their flags
+    // reference-rounding form: tiles whose WENO weight operands left the
+    // range of the unscaled division, for the IEEE-division launch that
+    // follows: a flag per tile (block-linear index, so a tile is queued
+    // once), the queue, and (queue length, blocks done) of that launch
     unsigned* fix;
+    unsigned* fixlist;
+    unsigned* fixctl;
 };
 
 // dF layout per direction: component-major, interface index t = 0..n along the
@@ -317,22 +329,14 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
 template <typename R, int MODE>
 constexpr int sweep_waves() { return (sizeof(R) == 4 && MODE != SW_COLLECT) ? HOWMHD_ROUTED_WAVES : HOWMHD_SWEEP_WAVES; }
 
-// DM (DivMode) of the WENO weights: the FMA forms and the reference-rounding
-// recompute launch divide with IEEE divisions; the reference-rounding sweep
-// proper with div_recip/div_with, flagging its block in A.fix where an
-// operand leaves their range.  A DIV_IEEE launch of the reference-rounding
-// form only runs the blocks whose flag is set.
+// One tile (bx, by, bz) of the sweep grid (sweep_grid), block-linear index
+// blk.  DM (DivMode) of the WENO weights: the FMA forms and the
+// reference-rounding recompute launch divide with IEEE divisions; the
+// reference-rounding sweep proper with div_recip/div_with, queueing its tile
+// for the recompute launch where an operand leaves their range.
 template <typename R, int DIR, int MODE, int EX, int DM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, MODE>()))) void k_sweep(
-    SweepArgs<R> A) {
-    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
-    static_assert(EX || DM == DIV_IEEE, "the FMA forms divide with IEEE divisions");
+__device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by, int bz, long long blk) {
     using T = SweepTile<DIR>;
-    constexpr bool RECOMPUTE = EX && DM == DIV_IEEE;
-    const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-    if constexpr (RECOMPUTE) {
-        if (A.fix[blk] == 0u) return;  // the same for the whole block
-    }
     __shared__ UF<R> luf[7][T::CELLS];
     __shared__ R lo[O_COUNT][T::CELLS];
     // |B| of each cell for the indicator features (routed and sample sweeps)
@@ -346,14 +350,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
     // block origin
     int t0, x0 = 0, y0 = 0, z0 = 0;
     if constexpr (DIR == 0) {
-        t0 = tbeg + blockIdx.x * T::LANES;
-        y0 = blockIdx.y * T::ROWS;
-        z0 = blockIdx.z;
+        t0 = tbeg + bx * T::LANES;
+        y0 = by * T::ROWS;
+        z0 = bz;
     } else {
-        x0 = blockIdx.x * T::LANES;
-        t0 = tbeg + blockIdx.y * T::ROWS;
-        if constexpr (DIR == 1) z0 = blockIdx.z;
-        else y0 = blockIdx.z;
+        x0 = bx * T::LANES;
+        t0 = tbeg + by * T::ROWS;
+        if constexpr (DIR == 1) z0 = bz;
+        else y0 = bz;
     }
 
     // ---- phase 1: cell values of the tile (stencil halo included) into LDS.
@@ -407,9 +411,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
         }
     }
     __syncthreads();
-    if constexpr (RECOMPUTE) {
-        if (threadIdx.x == 0) A.fix[blk] = 0u;  // every thread has read it
-    }
 
     // ---- phase 2: one interface per thread
     int lane, row, t, x, y, z;
@@ -475,10 +476,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
     if (MODE != SW_ROUTED || flagged) {
         unsigned long long bad = 0ull;
         weno_interface<R, DIR, MODE, DM>(luf, lo, row, lane, R(A.ph.gam), dF, maxdev, bad);
-        // a WENO weight operand beyond the unscaled-division range: the block
-        // is recomputed with IEEE divisions by the launch that follows
+        // a WENO weight operand beyond the unscaled-division range: the tile
+        // goes (once) on the list of the IEEE-division launch that follows
         if constexpr (EX && DM == DIV_UNSCALED) {
-            if (bad) A.fix[blk] = 1u;
+            if (bad && atomicExch(&A.fix[blk], 1u) == 0u) A.fixlist[atomicAdd(&A.fixctl[0], 1u)] = (unsigned)blk;
         }
     } else {
         // linear upwind-5 with one LLF speed: the largest |v_n| + c_f over the
@@ -542,6 +543,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
 struct CtFlags {
     int use_x, use_y, use_z, use_Ox, use_Oy, use_Oz, active;
 };
+
+// The sweep along DIR.  Reference rounding, DM = DIV_IEEE: the recompute
+// launch (a few blocks) takes the tiles queued by the sweep proper
+// (A.fixlist, A.fixctl[0] of them), clears their flags, and the last block
+// to finish empties the queue; with an empty queue every block exits at once.
+template <typename R, int DIR, int MODE, int EX, int DM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, MODE>()))) void k_sweep(
+    SweepArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
+    static_assert(EX || DM == DIV_IEEE, "the FMA forms divide with IEEE divisions");
+    if constexpr (EX && DM == DIV_IEEE) {
+        const unsigned n = A.fixctl[0];
+        const dim3 gr = sweep_grid(A.g, DIR);
+        for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+            const unsigned b = A.fixlist[i];
+            __syncthreads();  // the previous tile's LDS is free
+            sweep_tile<R, DIR, MODE, EX, DM>(A, (int)(b % gr.x), (int)((b / gr.x) % gr.y), (int)(b / (gr.x * gr.y)), b);
+            if (threadIdx.x == 0) A.fix[b] = 0u;
+        }
+        __syncthreads();  // every thread of the block has read the count
+        if (threadIdx.x == 0 && atomicAdd(&A.fixctl[1], 1u) == gridDim.x - 1) {
+            A.fixctl[0] = 0u;  // every block has read the count
+            A.fixctl[1] = 0u;
+        }
+    } else {
+        const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+        sweep_tile<R, DIR, MODE, EX, DM>(A, blockIdx.x, blockIdx.y, blockIdx.z, blk);
+    }
+}
 
 template <typename R>
 struct EmfArgs {

@@ -33,25 +33,28 @@ inline RK rk_coeff(int iw) {
 }
 
 template <typename R, int MODE, int DM>
-void launch_sweep_dm(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
-    const dim3 grid = sweep_grid(c->g, d);
+void launch_sweep_dm(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a, dim3 grid) {
     if (d == 0) hipLaunchKernelGGL((k_sweep<R, 0, MODE, EX, DM>), grid, dim3(256), 0, c->stream, a);
     else if (d == 1) hipLaunchKernelGGL((k_sweep<R, 1, MODE, EX, DM>), grid, dim3(256), 0, c->stream, a);
     else hipLaunchKernelGGL((k_sweep<R, 2, MODE, EX, DM>), grid, dim3(256), 0, c->stream, a);
 }
 
+// Blocks of the IEEE-division recompute launch of the reference-rounding form.
+constexpr unsigned kRecomputeBlocks = 64;
+
 // The sweep along d.  Reference rounding: the sweep with unscaled WENO
-// divisions, then the IEEE-division launch for the blocks it flagged (most
-// blocks exit at once; no counting there).
+// divisions, then the IEEE-division launch for the tiles it queued (it exits
+// at once when there are none; no counting there).
 template <typename R, int MODE>
 void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
+    const dim3 grid = sweep_grid(c->g, d);
     if constexpr (EX) {
-        launch_sweep_dm<R, MODE, DIV_UNSCALED>(c, d, a);
+        launch_sweep_dm<R, MODE, DIV_UNSCALED>(c, d, a, grid);
         SweepArgs<R> b = a;
         b.stats = nullptr;
-        launch_sweep_dm<R, MODE, DIV_IEEE>(c, d, b);
+        launch_sweep_dm<R, MODE, DIV_IEEE>(c, d, b, dim3(kRecomputeBlocks));
     } else {
-        launch_sweep_dm<R, MODE, DIV_IEEE>(c, d, a);
+        launch_sweep_dm<R, MODE, DIV_IEEE>(c, d, a, grid);
     }
 }
 
@@ -69,6 +72,8 @@ SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     a.mlp = c->mlp;
     a.stats = c->route_now ? c->iwave : nullptr;
     a.fix = c->fixflag;
+    a.fixlist = c->fixlist;
+    a.fixctl = c->fixctl;
     return a;
 }
 
