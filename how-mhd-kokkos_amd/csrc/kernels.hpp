@@ -113,6 +113,10 @@ template <typename R>
 struct SweepArgs {
     const double* q;    // source stage, 8 arrays of ncell
     R* dF;              // 7 x n_if planes (see dF_index)
+    // constrained transport active: components 4, 5 of dF (the
+    // transverse-B fluxes) feed only the CT terms, not k_combine, and are
+    // not stored
+    int ct_active;
     R* ct_a;            // CT term "bsy": fsy (x), gsz (y), hsx (z)
     R* ct_b;            // CT term "bsz": fsz (x), gsx (y), hsy (z)
     Geom g;
@@ -504,7 +508,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
     }
 
 #pragma unroll
-    for (int c = 0; c < 7; ++c) A.dF[dF_index<DIR>(g, c, t, x, y, z)] = dF[c];
+    for (int c = 0; c < 7; ++c)
+        if (!(A.ct_active && (c == 4 || c == 5))) A.dF[dF_index<DIR>(g, c, t, x, y, z)] = dF[c];
 
     // CT flux terms, src/ssprk.cpp:967-994 (cells k = 1..4; the -0.0 weights
     // are kept as written)
@@ -789,11 +794,17 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     cell_of(g, i, x, y, z);
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const StageCoef& s = A.s;
+    // With CT active the cell-centred B (m = 4..6) is replaced by the face
+    // interpolation below, so its flux update is not formed (its dF
+    // components are not stored either).
+    const bool ct = A.f.active;
+    auto upd = [&](int m) { return !(ct && m >= 4 && m <= 6); };
     double v[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const long long o = (long long)m * g.ncell + i;
-        v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
+        v[m] = 0.0;
+        if (upd(m)) v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
     }
     // slot of each q variable in the dF of each sweep (-1: the normal field)
     // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
@@ -807,6 +818,7 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
         const double dt = s.dtdx;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
+            if (!upd(m)) continue;
             double d = 0.0;
             if (sx[m] >= 0)
                 d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
@@ -818,6 +830,7 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
         const double dt = s.dtdy;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
+            if (!upd(m)) continue;
             double d = 0.0;
             if (sy[m] >= 0)
                 d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] - (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
@@ -829,6 +842,7 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
         const double dt = s.dtdz;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
+            if (!upd(m)) continue;
             double d = 0.0;
             if (sz[m] >= 0)
                 d = -dt * ((double)A.dFz[dF_index<2>(g, sz[m], tr, x, y, z)] - (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, z)]);

@@ -69,6 +69,7 @@ SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     if (d == 1) { a.ct_a = (R*)c->ct[3]; a.ct_b = (R*)c->ct[2]; }
     a.g = c->g;
     a.ph = c->ph;
+    a.ct_active = c->ctf.active;
     a.mlp = c->mlp;
     a.stats = c->route_now ? c->iwave : nullptr;
     a.fix = c->fixflag;
