@@ -34,6 +34,10 @@ struct howmhd_gpu_ctx {
     void* O2[3] = {};
     void* O[3] = {};
     unsigned long long* dmax = nullptr;
+    // CFL maxima per block of the last stage-5 combination (3 per block of
+    // k_combine); cfl_fresh: they belong to the current q(0)
+    double* cflpart = nullptr;
+    bool cfl_fresh = false;
     // learned indicator: device weights, routing mode (0 off, 1 on with the
     // plain-WENO fallback, 2 on for every step), the routed sweeps' counters
     // per wavefront slot (nwave slots, cleared by each fold) and their

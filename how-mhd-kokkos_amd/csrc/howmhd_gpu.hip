@@ -215,6 +215,7 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
         ok = ok && alloc(&c->O[t], c->ncell * c->rsize);
     }
     ok = ok && alloc((void**)&c->dmax, 4 * sizeof(unsigned long long));
+    ok = ok && alloc((void**)&c->cflpart, 3 * (size_t)blocks_for(c->ncell, 256) * sizeof(double));
     ok = ok && alloc((void**)&c->mlp, sizeof(MlpDev));
     // one counter slot per wavefront of the largest sweep grid
     for (int d = 0; d < 3; ++d) {
@@ -249,6 +250,7 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
     for (int t = 0; t < 6; ++t)
         if (c->ct[t]) (void)hipFree(c->ct[t]);
     if (c->dmax) (void)hipFree(c->dmax);
+    if (c->cflpart) (void)hipFree(c->cflpart);
     if (c->mlp) (void)hipFree(c->mlp);
     if (c->istats) (void)hipFree(c->istats);
     if (c->iwave) (void)hipFree(c->iwave);
@@ -271,6 +273,7 @@ int howmhd_gpu_upload_q0(howmhd_gpu_ctx* c, const double* q0) {
                              c->stream));
     HCHECK(c, hipStreamSynchronize(c->stream));
     c->faces_ready = false;
+    c->cfl_fresh = false;
     return HOWMHD_OK;
 }
 
@@ -299,11 +302,18 @@ int howmhd_gpu_tstep(howmhd_gpu_ctx* c, double* dt) {
     if (!c || !dt) return HOWMHD_EINVAL;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->dmax, 3);
     HCHECK(c, hipGetLastError());
-    const unsigned nb = std::min<unsigned>(blocks_for(c->ncell, 256), 2048u);
     {
         TimedLaunch tl(c, HOWMHD_K_CFL);
-        hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, c->stream, (const double*)c->qbuf[c->qid[0]], c->g, c->ph,
-                           c->dmax);
+        if (c->cfl_fresh) {
+            // maxima of the last step's stage-5 combination, which wrote q(0)
+            const long long nblk = blocks_for(c->ncell, 256);
+            const unsigned nb = std::min<unsigned>(blocks_for((size_t)nblk, 256), 256u);
+            hipLaunchKernelGGL(k_cfl_fold, dim3(nb), dim3(256), 0, c->stream, (const double*)c->cflpart, nblk, c->dmax);
+        } else {
+            const unsigned nb = std::min<unsigned>(blocks_for(c->ncell, 256), 2048u);
+            hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, c->stream, (const double*)c->qbuf[c->qid[0]], c->g,
+                               c->ph, c->dmax);
+        }
     }
     HCHECK(c, hipGetLastError());
     unsigned long long h[3];
@@ -344,6 +354,7 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
             ++c->routed_steps;
         }
     }
+    c->cfl_fresh = false;
     for (int iw = 1; iw <= 5; ++iw) {
         const int rc = stage(c, iw, dt);
         if (rc) return rc;
@@ -367,6 +378,7 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
     // q(0) <- q(5), faces likewise: exchange the buffers of stages 0 and 5
     std::swap(c->qid[0], c->qid[1]);
     c->qid[5] = c->qid[1];
+    c->cfl_fresh = true;  // the stage-5 combination left the CFL maxima of the new q(0)
     return HOWMHD_OK;
 }
 

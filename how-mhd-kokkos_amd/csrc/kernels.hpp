@@ -760,6 +760,50 @@ __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
     A.bo[2 * g.ncell + i] = bz;
 }
 
+// CFL speeds of one cell (src/tstep.cpp:28-83): the running maxima of
+// |v_d| + c_fast,d from the conserved values, with the reference's rounding
+// in every unit (no contraction here).
+__device__ __forceinline__ void cfl_cell(const double* q, const Phys& ph, double& m0, double& m1, double& m2) {
+#pragma clang fp contract(off)
+    const double gam = ph.gam;
+    const double DD = q[0], Mx = q[1], My = q[2], Mz = q[3], Bx = q[4], By = q[5], Bz = q[6], EE = q[7];
+    const double rho = fmax(DD, ph.rhomin);
+    const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
+    const double vv2 = vx * vx + vy * vy + vz * vz;
+    const double BB2 = Bx * Bx + By * By + Bz * Bz;
+    const double pg = fmax(ph.pgmin, (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2)));
+    const double bbn2 = BB2 / rho;
+    const double cs2 = fmax(0.0, gam * pg / rho);
+    const double bnx2 = Bx * Bx / rho, bny2 = By * By / rho, bnz2 = Bz * Bz / rho;
+    const double rootx = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+    const double rooty = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bny2 * cs2));
+    const double rootz = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnz2 * cs2));
+    const double lfx = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootx)));
+    const double lfy = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rooty)));
+    const double lfz = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootz)));
+    m0 = fmax(m0, fabs(vx) + lfx);
+    m1 = fmax(m1, fabs(vy) + lfy);
+    m2 = fmax(m2, fabs(vz) + lfz);
+}
+
+// Maxima of m0..m2 over the 256 threads of the block (all must call it), in
+// out[0..2] of thread 0.
+__device__ __forceinline__ void block_max3(double m0, double m1, double m2, double* out) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        m0 = fmax(m0, __shfl_xor(m0, off, 64));
+        m1 = fmax(m1, __shfl_xor(m1, off, 64));
+        m2 = fmax(m2, __shfl_xor(m2, off, 64));
+    }
+    __shared__ double red[3][4];
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = m0; red[1][w] = m1; red[2][w] = m2; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[d] = fmax(fmax(red[d][0], red[d][1]), fmax(red[d][2], red[d][3]));
+}
+
 template <typename R>
 struct CombineArgs {
     const double* q0;
@@ -773,6 +817,9 @@ struct CombineArgs {
     CtFlags f;
     StageCoef s;
     Phys ph;
+    // stage 5: per-block CFL maxima of the new q(0) (3 doubles per block),
+    // which the next tstep reduces instead of reading q(0) again; else null
+    double* cfl;
 };
 
 // 6th-order face -> centre interpolation of src/fluxct.cpp:891-896:
@@ -789,104 +836,108 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
-    if (i >= g.ncell) return;
-    int x, y, z;
-    cell_of(g, i, x, y, z);
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const StageCoef& s = A.s;
-    // With CT active the cell-centred B (m = 4..6) is replaced by the face
-    // interpolation below, so its flux update is not formed (its dF
-    // components are not stored either).
-    const bool ct = A.f.active;
-    auto upd = [&](int m) { return !(ct && m >= 4 && m <= 6); };
-    double v[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        const long long o = (long long)m * g.ncell + i;
-        v[m] = 0.0;
-        if (upd(m)) v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
-    }
-    // slot of each q variable in the dF of each sweep (-1: the normal field)
-    // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
-    // y: rho->0, Mx->3, My->1, Mz->2, Bx->5, By->-, Bz->4, E->6
-    // z: rho->0, Mx->2, My->3, Mz->1, Bx->4, By->5, Bz->-, E->6
-    constexpr int sx[8] = {0, 1, 2, 3, -1, 4, 5, 6};
-    constexpr int sy[8] = {0, 3, 1, 2, 5, -1, 4, 6};
-    constexpr int sz[8] = {0, 2, 3, 1, 4, 5, -1, 6};
-    if (nx > 1) {
-        const int tl = (x == 0 && g.per[0]) ? nx : x, tr = x + 1;
-        const double dt = s.dtdx;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;  // CFL maxima (stage 5)
+    if (i < g.ncell) {
+        int x, y, z;
+        cell_of(g, i, x, y, z);
+        const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+        const StageCoef& s = A.s;
+        // With CT active the cell-centred B (m = 4..6) is replaced by the face
+        // interpolation below, so its flux update is not formed (its dF
+        // components are not stored either).
+        const bool ct = A.f.active;
+        auto upd = [&](int m) { return !(ct && m >= 4 && m <= 6); };
+        double v[8];
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
-            if (!upd(m)) continue;
-            double d = 0.0;
-            if (sx[m] >= 0)
-                d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
-            v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
+            const long long o = (long long)m * g.ncell + i;
+            v[m] = 0.0;
+            if (upd(m)) v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
         }
-    }
-    if (ny > 1) {
-        const int tl = (y == 0 && g.per[1]) ? ny : y, tr = y + 1;
-        const double dt = s.dtdy;
+        // slot of each q variable in the dF of each sweep (-1: the normal field)
+        // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
+        // y: rho->0, Mx->3, My->1, Mz->2, Bx->5, By->-, Bz->4, E->6
+        // z: rho->0, Mx->2, My->3, Mz->1, Bx->4, By->5, Bz->-, E->6
+        constexpr int sx[8] = {0, 1, 2, 3, -1, 4, 5, 6};
+        constexpr int sy[8] = {0, 3, 1, 2, 5, -1, 4, 6};
+        constexpr int sz[8] = {0, 2, 3, 1, 4, 5, -1, 6};
+        if (nx > 1) {
+            const int tl = (x == 0 && g.per[0]) ? nx : x, tr = x + 1;
+            const double dt = s.dtdx;
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            if (!upd(m)) continue;
-            double d = 0.0;
-            if (sy[m] >= 0)
-                d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] - (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
-            v[m] = v[m] + d;
+            for (int m = 0; m < 8; ++m) {
+                if (!upd(m)) continue;
+                double d = 0.0;
+                if (sx[m] >= 0)
+                    d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
+                v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
+            }
         }
-    }
-    if (nz > 1) {
-        const int tl = (z == 0 && g.per[2]) ? nz : z, tr = z + 1;
-        const double dt = s.dtdz;
+        if (ny > 1) {
+            const int tl = (y == 0 && g.per[1]) ? ny : y, tr = y + 1;
+            const double dt = s.dtdy;
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            if (!upd(m)) continue;
-            double d = 0.0;
-            if (sz[m] >= 0)
-                d = -dt * ((double)A.dFz[dF_index<2>(g, sz[m], tr, x, y, z)] - (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, z)]);
-            v[m] = v[m] + d;
+            for (int m = 0; m < 8; ++m) {
+                if (!upd(m)) continue;
+                double d = 0.0;
+                if (sy[m] >= 0)
+                    d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] - (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
+                v[m] = v[m] + d;
+            }
         }
-    }
-    if (A.f.active) {
-        const double* bx = A.bo;
-        const double* by = A.bo + g.ncell;
-        const double* bz = A.bo + 2 * g.ncell;
-        if (A.f.use_x) {
-            const int p = g.per[0];
-            v[4] = i6(bx[lin(g, bmap(x - 3, nx, p), y, z)], bx[lin(g, bmap(x - 2, nx, p), y, z)], bx[lin(g, bmap(x - 1, nx, p), y, z)],
-                      bx[i], bx[lin(g, bmap(x + 1, nx, p), y, z)], bx[lin(g, bmap(x + 2, nx, p), y, z)]);
-        } else {
-            v[4] = bx[i];
-        }
-        if (A.f.use_y) {
-            const int p = g.per[1];
-            v[5] = i6(by[lin(g, x, bmap(y - 3, ny, p), z)], by[lin(g, x, bmap(y - 2, ny, p), z)], by[lin(g, x, bmap(y - 1, ny, p), z)],
-                      by[i], by[lin(g, x, bmap(y + 1, ny, p), z)], by[lin(g, x, bmap(y + 2, ny, p), z)]);
-        } else {
-            v[5] = by[i];
-        }
-        if (A.f.use_z) {
-            const int p = g.per[2];
-            v[6] = i6(bz[lin(g, x, y, bmap(z - 3, nz, p))], bz[lin(g, x, y, bmap(z - 2, nz, p))], bz[lin(g, x, y, bmap(z - 1, nz, p))],
-                      bz[i], bz[lin(g, x, y, bmap(z + 1, nz, p))], bz[lin(g, x, y, bmap(z + 2, nz, p))]);
-        } else {
-            v[6] = bz[i];
-        }
-    }
-    // prot
-    const double gam = A.ph.gam;
-    double rho = fmax(v[0], A.ph.rhomin);
-    const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
-    const double vv2 = vx * vx + vy * vy + vz * vz;
-    const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
-    double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
-    pg = fmax(pg, A.ph.pgmin);
-    v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
-    v[0] = rho;
+        if (nz > 1) {
+            const int tl = (z == 0 && g.per[2]) ? nz : z, tr = z + 1;
+            const double dt = s.dtdz;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
+            for (int m = 0; m < 8; ++m) {
+                if (!upd(m)) continue;
+                double d = 0.0;
+                if (sz[m] >= 0)
+                    d = -dt * ((double)A.dFz[dF_index<2>(g, sz[m], tr, x, y, z)] - (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, z)]);
+                v[m] = v[m] + d;
+            }
+        }
+        if (A.f.active) {
+            const double* bx = A.bo;
+            const double* by = A.bo + g.ncell;
+            const double* bz = A.bo + 2 * g.ncell;
+            if (A.f.use_x) {
+                const int p = g.per[0];
+                v[4] = i6(bx[lin(g, bmap(x - 3, nx, p), y, z)], bx[lin(g, bmap(x - 2, nx, p), y, z)], bx[lin(g, bmap(x - 1, nx, p), y, z)],
+                          bx[i], bx[lin(g, bmap(x + 1, nx, p), y, z)], bx[lin(g, bmap(x + 2, nx, p), y, z)]);
+            } else {
+                v[4] = bx[i];
+            }
+            if (A.f.use_y) {
+                const int p = g.per[1];
+                v[5] = i6(by[lin(g, x, bmap(y - 3, ny, p), z)], by[lin(g, x, bmap(y - 2, ny, p), z)], by[lin(g, x, bmap(y - 1, ny, p), z)],
+                          by[i], by[lin(g, x, bmap(y + 1, ny, p), z)], by[lin(g, x, bmap(y + 2, ny, p), z)]);
+            } else {
+                v[5] = by[i];
+            }
+            if (A.f.use_z) {
+                const int p = g.per[2];
+                v[6] = i6(bz[lin(g, x, y, bmap(z - 3, nz, p))], bz[lin(g, x, y, bmap(z - 2, nz, p))], bz[lin(g, x, y, bmap(z - 1, nz, p))],
+                          bz[i], bz[lin(g, x, y, bmap(z + 1, nz, p))], bz[lin(g, x, y, bmap(z + 2, nz, p))]);
+            } else {
+                v[6] = bz[i];
+            }
+        }
+        // prot
+        const double gam = A.ph.gam;
+        double rho = fmax(v[0], A.ph.rhomin);
+        const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
+        const double vv2 = vx * vx + vy * vy + vz * vz;
+        const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
+        double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
+        pg = fmax(pg, A.ph.pgmin);
+        v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
+        v[0] = rho;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
+        if (A.cfl) cfl_cell(v, A.ph, m0, m1, m2);
+    }
+    if (A.cfl) block_max3(m0, m1, m2, A.cfl + 3 * (long long)blockIdx.x);
 }
 
 }  // namespace howmhd

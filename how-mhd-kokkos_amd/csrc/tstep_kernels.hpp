@@ -13,44 +13,33 @@ namespace howmhd {
 // so the block maxima are combined with a 64-bit atomicMax.
 __global__ __launch_bounds__(256) void k_cfl(const double* q, Geom g, Phys ph, unsigned long long* out) {
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    const double gam = ph.gam;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < g.ncell; i += (long long)gridDim.x * 256) {
-        const double DD = q[i], Mx = q[g.ncell + i], My = q[2 * g.ncell + i], Mz = q[3 * g.ncell + i];
-        const double Bx = q[4 * g.ncell + i], By = q[5 * g.ncell + i], Bz = q[6 * g.ncell + i], EE = q[7 * g.ncell + i];
-        const double rho = fmax(DD, ph.rhomin);
-        const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
-        const double vv2 = vx * vx + vy * vy + vz * vz;
-        const double BB2 = Bx * Bx + By * By + Bz * Bz;
-        const double pg = fmax(ph.pgmin, (gam - 1.0) * (EE - 0.5 * (rho * vv2 + BB2)));
-        const double bbn2 = BB2 / rho;
-        const double cs2 = fmax(0.0, gam * pg / rho);
-        const double bnx2 = Bx * Bx / rho, bny2 = By * By / rho, bnz2 = Bz * Bz / rho;
-        const double rootx = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
-        const double rooty = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bny2 * cs2));
-        const double rootz = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnz2 * cs2));
-        const double lfx = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootx)));
-        const double lfy = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rooty)));
-        const double lfz = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + rootz)));
-        m0 = fmax(m0, fabs(vx) + lfx);
-        m1 = fmax(m1, fabs(vy) + lfy);
-        m2 = fmax(m2, fabs(vz) + lfz);
-    }
-    // wavefront reduction (64 lanes), then across the 4 wavefronts
+        double c[8];
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        m0 = fmax(m0, __shfl_xor(m0, off, 64));
-        m1 = fmax(m1, __shfl_xor(m1, off, 64));
-        m2 = fmax(m2, __shfl_xor(m2, off, 64));
+        for (int m = 0; m < 8; ++m) c[m] = q[m * g.ncell + i];
+        cfl_cell(c, ph, m0, m1, m2);
     }
-    __shared__ double red[3][4];
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { red[0][w] = m0; red[1][w] = m1; red[2][w] = m2; }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const int d = threadIdx.x;
-        const double r = fmax(fmax(red[d][0], red[d][1]), fmax(red[d][2], red[d][3]));
-        atomicMax(&out[d], (unsigned long long)__double_as_longlong(r));
+    __shared__ double bm[3];
+    block_max3(m0, m1, m2, bm);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) atomicMax(&out[d], (unsigned long long)__double_as_longlong(bm[d]));
+}
+
+// The same maxima from the per-block maxima that the stage-5 combination
+// of the last step wrote (n blocks x 3).
+__global__ __launch_bounds__(256) void k_cfl_fold(const double* part, long long n, unsigned long long* out) {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (long long b = (long long)blockIdx.x * 256 + threadIdx.x; b < n; b += (long long)gridDim.x * 256) {
+        m0 = fmax(m0, part[3 * b]);
+        m1 = fmax(m1, part[3 * b + 1]);
+        m2 = fmax(m2, part[3 * b + 2]);
     }
+    __shared__ double bm[3];
+    block_max3(m0, m1, m2, bm);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) atomicMax(&out[d], (unsigned long long)__double_as_longlong(bm[d]));
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {

@@ -108,6 +108,24 @@ def test_fp64_bitwise_large_weno_operands(problem, kw, require_gpu):
     assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp64fast", "fp32"])
+def test_dt_from_fused_cfl_maxima(precision, require_gpu):
+    """After a step, tstep takes the CFL maxima that the stage-5 combination
+    computed while writing q(0); in every form dt is the reference's dt of
+    that q(0) bit for bit (the restatement's tstep on the downloaded state)."""
+    run, q0, p = _setup("mhd_blast3d", dict(nx=16, ny=12, nz=20, cour=0.4))
+    p.precision = H.PREC[precision]
+    gpu = H.GpuSolver(p)
+    gpu.upload_q0(q0)
+    gpu.init_faces()
+    for step in range(3):
+        dt = gpu.tstep()
+        gpu.ssprk(dt)
+        ref = O.RefSolver(p)
+        ref.upload_q0(gpu.q0())
+        assert gpu.tstep() == ref.tstep(), f"{precision}: dt differs after step {step}"
+
+
 SMOOTH = {"smooth16", "cpaw", "mhdblast_open"}
 
 
