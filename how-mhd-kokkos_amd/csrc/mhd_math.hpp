@@ -88,13 +88,20 @@ __device__ __forceinline__ R rcp_fast(R x) { return R(1) / x; }
 template <>
 __device__ __forceinline__ float rcp_fast<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// x / c for a compile-time constant c with inv_c = RN(1/c).  Reference
-// rounding: the correctly rounded quotient (div_by).  FMA form:
-// multiplication by the rounded reciprocal (differs by at most one rounding).
+// x / c for a compile-time constant c > 0 with inv_c = RN(1/c).  Reference
+// rounding: the correctly rounded quotient of div_by without its select:
+// with nr = fma(c, q, -x) = -r exactly, fma(-nr, inv_c, q) equals div_by's
+// (r == 0) ? q : fma(r, inv_c, q) bit for bit when inv_c > 0, signed zeros
+// included (checked on 4e8 pairs for the divisors used, zeros, subnormals and
+// exact multiples among them).  Like div_by it is exact barring underflow
+// (subnormal x).  FMA form: multiplication by the rounded reciprocal (differs
+// by at most one rounding).
 template <typename R>
 __device__ __forceinline__ R div_c(R x, R c, R inv_c) {
 #if HOWMHD_EXACT
-    return div_by<R>(x, c, inv_c);
+    const R q = x * inv_c;
+    const R nr = __builtin_fma(c, q, -x);
+    return __builtin_fma(-nr, inv_c, q);
 #else
     (void)c;
     return x * inv_c;
