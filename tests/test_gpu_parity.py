@@ -126,6 +126,27 @@ def test_dt_from_fused_cfl_maxima(precision, require_gpu):
         assert gpu.tstep() == ref.tstep(), f"{precision}: dt differs after step {step}"
 
 
+def test_fp64_bitwise_bench_workload(require_gpu):
+    """The benchmark workload itself, MHD blast at 256^3: one full step
+    (tstep + SSPRK(5,4)) of the GPU path equals the restatement bit for bit
+    (q, faces and dt)."""
+    run, q0, p = _setup("mhd_blast3d", dict(nx=256, ny=256, nz=256, cour=0.4))
+    print("[parity fp64] mhd_blast3d 256^3, 1 step", flush=True)
+    ref = O.RefSolver(p)
+    gpu = H.GpuSolver(p)
+    for s in (ref, gpu):
+        s.upload_q0(q0)
+        s.init_faces()
+    del q0
+    dt = ref.tstep()
+    assert gpu.tstep() == dt
+    gpu.ssprk(dt)
+    ref.ssprk(dt)
+    assert _bitwise_mismatch(gpu.q0(), ref.q0()) == 0
+    assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
+    assert gpu.tstep() == ref.tstep()
+
+
 SMOOTH = {"smooth16", "cpaw", "mhdblast_open"}
 
 
