@@ -701,16 +701,47 @@ __device__ __forceinline__ double d4(double om3, double om2, double om1, double 
 
 // fluxct_update_bfield, src/fluxct.cpp:850-877, with the stage initialisation
 // of copy_bfield_stage / prepare_bfield_stage5 (111-135) folded in.
+// Block = FACE_TX x FACE_TY cells of one z plane.  The in-plane stencils
+// (Ox along x and y, Oy along y, Oz along x) are read from LDS copies of the
+// tile with its 3-cell halo (each value read once per block instead of six
+// times through L2); the z stencils of Oy and Oz are read from memory.
+constexpr int FACE_TX = 32, FACE_TY = 8;
 template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    constexpr int HX = FACE_TX + 5, HY = FACE_TY + 5;
+    __shared__ R sOx[HY][HX];       // rows y0-3 .. y0+TY+1, columns x0-3 .. x0+TX+1
+    __shared__ R sOy[HY][FACE_TX];  // rows y0-3 .. y0+TY+1
+    __shared__ R sOz[FACE_TY][HX];  // columns x0-3 .. x0+TX+1
     const Geom& g = A.g;
-    if (i >= g.ncell) return;
-    int x, y, z;
-    cell_of(g, i, x, y, z);
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int px = g.per[0], py = g.per[1], pz = g.per[2];
+    const int x0 = blockIdx.x * FACE_TX, y0 = blockIdx.y * FACE_TY, z = blockIdx.z;
+    const int tid = threadIdx.x;
     const CtFlags& f = A.f;
+    // halo coordinates beyond the stencil reach of any valid cell are clamped
+    // into the range bmap covers (those entries are never read)
+    auto hx = [&](int c) { const int a = x0 - 3 + c; return bmap(a > nx + 2 ? nx + 2 : a, nx, px); };
+    auto hy = [&](int r) { const int a = y0 - 3 + r; return bmap(a > ny + 2 ? ny + 2 : a, ny, py); };
+    if (f.active) {
+        if (f.use_Ox)
+            for (int k = tid; k < HY * HX; k += 256) sOx[k / HX][k % HX] = A.Ox[lin(g, hx(k % HX), hy(k / HX), z)];
+        if (f.use_Oy)
+            for (int k = tid; k < HY * FACE_TX; k += 256) {
+                const int c = k % FACE_TX, xx = x0 + c < nx ? x0 + c : nx - 1;
+                sOy[k / FACE_TX][c] = A.Oy[lin(g, xx, hy(k / FACE_TX), z)];
+            }
+        if (f.use_Oz)
+            for (int k = tid; k < FACE_TY * HX; k += 256) {
+                const int r = k / HX, yy = y0 + r < ny ? y0 + r : ny - 1;
+                sOz[r][k % HX] = A.Oz[lin(g, hx(k % HX), yy, z)];
+            }
+    }
+    __syncthreads();
+    const int tx = tid % FACE_TX, ty = tid / FACE_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= nx || y >= ny) return;
+    const long long i = lin(g, x, y, z);
     const StageCoef& s = A.s;
     double binit[3];
 #pragma unroll
@@ -718,40 +749,39 @@ __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
         const long long o = (long long)c * g.ncell + i;
         binit[c] = s.stage5 ? A.b0[o] + s.c4 * A.b2[o] + s.c5 * A.b3[o] : A.b0[o];
     }
-    auto Oat = [&](const R* O, int xx, int yy, int zz) -> double { return (double)O[lin(g, xx, yy, zz)]; };
-    auto dy_of = [&](const R* O) -> double {
-        const int p = g.per[1];
-        return d4(Oat(O, x, bmap(y - 3, ny, p), z), Oat(O, x, bmap(y - 2, ny, p), z), Oat(O, x, bmap(y - 1, ny, p), z),
-                  Oat(O, x, y, z), Oat(O, x, bmap(y + 1, ny, p), z), Oat(O, x, bmap(y + 2, ny, p), z));
-    };
     auto dz_of = [&](const R* O) -> double {
-        const int p = g.per[2];
-        return d4(Oat(O, x, y, bmap(z - 3, nz, p)), Oat(O, x, y, bmap(z - 2, nz, p)), Oat(O, x, y, bmap(z - 1, nz, p)),
-                  Oat(O, x, y, z), Oat(O, x, y, bmap(z + 1, nz, p)), Oat(O, x, y, bmap(z + 2, nz, p)));
+        auto at = [&](int zz) -> double { return (double)O[lin(g, x, y, zz)]; };
+        return d4(at(bmap(z - 3, nz, pz)), at(bmap(z - 2, nz, pz)), at(bmap(z - 1, nz, pz)), at(z),
+                  at(bmap(z + 1, nz, pz)), at(bmap(z + 2, nz, pz)));
     };
-    auto dx_of = [&](const R* O) -> double {
-        const int p = g.per[0];
-        return d4(Oat(O, bmap(x - 3, nx, p), y, z), Oat(O, bmap(x - 2, nx, p), y, z), Oat(O, bmap(x - 1, nx, p), y, z),
-                  Oat(O, x, y, z), Oat(O, bmap(x + 1, nx, p), y, z), Oat(O, bmap(x + 2, nx, p), y, z));
+    auto col6 = [&](const R (*t)[HX], int c) -> double {  // rows y-3..y+2 of column c
+        return d4((double)t[ty][c], (double)t[ty + 1][c], (double)t[ty + 2][c], (double)t[ty + 3][c],
+                  (double)t[ty + 4][c], (double)t[ty + 5][c]);
+    };
+    auto row6 = [&](const R* r) -> double {  // columns x-3..x+2 of a tile row
+        return d4((double)r[tx], (double)r[tx + 1], (double)r[tx + 2], (double)r[tx + 3], (double)r[tx + 4],
+                  (double)r[tx + 5]);
     };
     double bx = binit[0], by = binit[1], bz = binit[2];
     if (f.active) {
         if (f.use_Ox || f.use_Oz) {
             double dOx = 0.0, dOz = 0.0;
-            if (f.use_Ox) dOx = dy_of(A.Ox);
+            if (f.use_Ox) dOx = col6(sOx, tx + 3);
             if (f.use_Oz) dOz = dz_of(A.Oz);
             bx = s.k1 * binit[0] + s.k3 * A.bp[i] - s.dtdy * dOx + s.dtdz * dOz;
         }
         if (f.use_Oy || f.use_Ox) {
             double dOy = 0.0, dOx = 0.0;
             if (f.use_Oy) dOy = dz_of(A.Oy);
-            if (f.use_Ox) dOx = dx_of(A.Ox);
+            if (f.use_Ox) dOx = row6(sOx[ty + 3]);
             by = s.k1 * binit[1] + s.k3 * A.bp[g.ncell + i] - s.dtdz * dOy + s.dtdx * dOx;
         }
         if (f.use_Oz || f.use_Oy) {
             double dOz = 0.0, dOy = 0.0;
-            if (f.use_Oz) dOz = dx_of(A.Oz);
-            if (f.use_Oy) dOy = dy_of(A.Oy);
+            if (f.use_Oz) dOz = row6(sOz[ty]);
+            if (f.use_Oy)
+                dOy = d4((double)sOy[ty][tx], (double)sOy[ty + 1][tx], (double)sOy[ty + 2][tx], (double)sOy[ty + 3][tx],
+                         (double)sOy[ty + 4][tx], (double)sOy[ty + 5][tx]);
             bz = s.k1 * binit[2] + s.k3 * A.bp[2 * g.ncell + i] - s.dtdx * dOz + s.dtdy * dOy;
         }
     }

@@ -151,7 +151,8 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     fa.s = s;
     {
         TimedLaunch tl(c, HOWMHD_K_FACES);
-        hipLaunchKernelGGL((k_faces<R, EX>), dim3(nb), dim3(256), 0, c->stream, fa);
+        const dim3 fgrid(blocks_for(c->g.n[0], FACE_TX), blocks_for(c->g.n[1], FACE_TY), c->g.n[2]);
+        hipLaunchKernelGGL((k_faces<R, EX>), fgrid, dim3(256), 0, c->stream, fa);
     }
     HCHECK(c, hipGetLastError());
 
