@@ -39,13 +39,15 @@ METRIC = "zone-cycles/s per SSPRK(5,4) step at 256³; speedup vs plain FP64 at f
 FLOP_SWEEP_CELL = 2850.0
 FLOP_ZONE_STEP = 44000.0
 BYTES_ZONE_STEP = 1472.0
-# bytes per cell of each memory-bound kernel (FP64 path, compulsory reads+writes)
+# bytes per cell of each memory-bound kernel (FP64 path, compulsory reads+writes
+# of stages 2-4; CT active, so the transverse-B fluxes and q's cell-centred B
+# are not read by the combination)
 BYTES_CELL = {
     "emf2": 8 * (6 + 3),          # 6 CT flux arrays in, 3 EMF candidates out
     "emf": 8 * (3 + 3),
     "faces": 8 * (3 + 3 + 3),     # O in, previous + initial faces in, faces out
-    "combine": 8 * (21 + 8 + 8 + 3 + 8),  # 3x7 fluxes, q_init, q_prev, faces in; q out
-    "cfl": 8 * 8,
+    "combine": 8 * (15 + 5 + 5 + 3 + 8),  # 3x5 fluxes, q_init and q_prev without B, faces in; q out
+    "cfl": 24.0 / 256,            # tstep folds 3 maxima per 256-cell block of the stage-5 combination
 }
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
