@@ -250,6 +250,8 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
     }
         Eig<R> e;
     eig_scalars<R>(gam, u0, u1, e);
+    // the unscaled mode normalises the L rows with div_pos: needs 1/cs2 > 0
+    if constexpr (DM == DIV_UNSCALED) bad |= __ballot(!(e.nrm_half > R(0)));
 
     // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
     Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
@@ -281,8 +283,8 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
         // rows and columns here instead of all seven up front (98 live values)
         opaque(e);
         R La[7], Lb[7];
-        l_row<m0>(gam, e, La);
-        if constexpr (two) l_row<m1>(gam, e, Lb);
+        l_row<m0, R, DM>(gam, e, La);
+        if constexpr (two) l_row<m1, R, DM>(gam, e, Lb);
         R Fa[6], qa[6], Fb[6], qb[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -400,12 +402,13 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
             for (int m = 0; m < 8; ++m) qs[it][m] = A.q[(long long)kPerm[DIR][m] * g.ncell + idx];
         }
     }
+    bool cell_bad = false;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int s = tid + 256 * it;
         if (s < T::CELLS) {
             R out[C_NFIELD];
-            cell_values<R>(qs[it], A.ph.gam, A.ph.rhomin, A.ph.pgmin, out);
+            cell_values<R, DM>(qs[it], A.ph.gam, A.ph.rhomin, A.ph.pgmin, out, &cell_bad);
 #pragma unroll
             for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
 #pragma unroll
@@ -413,6 +416,12 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
             if constexpr (MODE != SW_PLAIN)
                 lbmag[s] = sqrtf((float)(out[C_BN] * out[C_BN] + out[C_BT1] * out[C_BT1] + out[C_BT2] * out[C_BT2]));
         }
+    }
+    // a cell without positive shared reciprocals: the tile goes (once) on the
+    // list of the IEEE-division launch that follows
+    if constexpr (EX && DM == DIV_UNSCALED) {
+        if (__ballot(cell_bad) && atomicExch(&A.fix[blk], 1u) == 0u)
+            A.fixlist[atomicAdd(&A.fixctl[0], 1u)] = (unsigned)blk;
     }
     __syncthreads();
 

@@ -69,6 +69,14 @@ __device__ __forceinline__ double div_with(double a, double b, double r) {
     return __builtin_fma(__builtin_fma(-b, q, a), r, q);
 }
 
+
+// 1/x in the FMA forms: FP32 takes the hardware reciprocal (v_rcp_f32,
+// within 1 ulp) instead of the IEEE division sequence; FP64 divides.
+template <typename R>
+__device__ __forceinline__ R rcp_fast(R x) { return R(1) / x; }
+template <>
+__device__ __forceinline__ float rcp_fast<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Division form of the reference-rounding WENO weights: IEEE divisions, or
 // div_recip/div_with, which hold for t_k = (eps + IS_k)^2 <= 1e200 (then
 // alpha_k in [1e-200, 6e20] and omega in (5e-222, 1]).  The unscaled form
@@ -81,12 +89,26 @@ enum DivMode { DIV_IEEE = 0, DIV_UNSCALED = 1 };
 constexpr uint32_t kUnscaledHi = 0x6974E718u;
 __device__ __forceinline__ uint32_t hi_word(double x) { return (uint32_t)((unsigned long long)__double_as_longlong(x) >> 32); }
 
-// 1/x in the FMA forms: FP32 takes the hardware reciprocal (v_rcp_f32,
-// within 1 ulp) instead of the IEEE division sequence; FP64 divides.
+// a / b for b > 0 from y = RN(1/b) > 0: div_by without its select.  With
+// nr = fma(b, q, -a) = -r exactly, fma(-nr, y, q) equals div_by's
+// (r == 0) ? q : fma(r, y, q) bit for bit whenever y > 0, signed zeros
+// included (checked on 4e8 pairs: random b > 0 and a, zeros, exact multiples,
+// all-ones significands); for y < 0 the two differ in the sign of a zero
+// quotient, so callers check y > 0.
 template <typename R>
-__device__ __forceinline__ R rcp_fast(R x) { return R(1) / x; }
-template <>
-__device__ __forceinline__ float rcp_fast<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ R div_pos(R a, R b, R y) {
+    const R q = a * y;
+    return __builtin_fma(-__builtin_fma(b, q, -a), y, q);
+}
+
+// Reference-rounding quotient by a shared reciprocal in division mode DM:
+// div_pos in the unscaled mode (the caller raises its flag when y <= 0), else
+// div_by.
+template <int DM, typename R>
+__device__ __forceinline__ R div_q(R a, R b, R y) {
+    if constexpr (DM == DIV_UNSCALED) return div_pos<R>(a, b, y);
+    else return div_by<R>(a, b, y);
+}
 
 // x / c for a compile-time constant c > 0 with inv_c = RN(1/c).  Reference
 // rounding: the correctly rounded quotient of div_by without its select:
@@ -129,9 +151,12 @@ struct CellOut {
     R f[C_NFIELD];
 };
 
-template <typename R>
+// DM: division mode of the reference-rounding form; in the unscaled mode
+// *bad is raised for a cell whose shared reciprocals are not positive (it is
+// then recomputed with div_by).
+template <typename R, int DM = DIV_IEEE>
 __device__ __forceinline__ void cell_values(const double* qs, double gam, double rhomin, double pgmin,
-                                            R* out) {
+                                            R* out, bool* bad = nullptr) {
     const double DD = qs[0], Mx = qs[1], My = qs[2], Mz = qs[3];
     const double Bx = qs[4], By = qs[5], Bz = qs[6];
     double EE = qs[7];
@@ -139,9 +164,10 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
 #if HOWMHD_EXACT
     // correctly rounded quotients from shared reciprocals (div_by)
     const double iDD = 1.0 / DD;
-    const double vx = div_by(Mx, DD, iDD);
-    const double vy = div_by(My, DD, iDD);
-    const double vz = div_by(Mz, DD, iDD);
+    if constexpr (DM == DIV_UNSCALED) *bad = *bad || !(iDD > 0.0);  // rho >= DD > 0 then
+    const double vx = div_q<DM>(Mx, DD, iDD);
+    const double vy = div_q<DM>(My, DD, iDD);
+    const double vz = div_q<DM>(Mz, DD, iDD);
 #else
     double irho = 1.0 / rho;
     const double vx = Mx * irho;
@@ -163,7 +189,7 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     }
 #if HOWMHD_EXACT
     const double irho = (rho == DD) ? iDD : 1.0 / rho;  // floored rho
-    const double HH = div_by(EE + pg, rho, irho);
+    const double HH = div_q<DM>(EE + pg, rho, irho);
 #else
     const double HH = (EE + pg) * irho;
 #endif
@@ -178,9 +204,9 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const R vdB = rvx * rBx + rvy * rBy + rvz * rBz;
 #if HOWMHD_EXACT
     static_assert(sizeof(R) == 8, "the reference-rounding form is FP64");
-    const R bbn2 = div_by<R>(rBB2, rrho, irho);
-    const R bnx2 = div_by<R>(rBx * rBx, rrho, irho);
-    const R cs2 = rmax(R(0), rgam * rabs(div_by<R>(rpg, rrho, irho)));
+    const R bbn2 = div_q<DM>(rBB2, rrho, irho);
+    const R bnx2 = div_q<DM>(rBx * rBx, rrho, irho);
+    const R cs2 = rmax(R(0), rgam * rabs(div_q<DM>(rpg, rrho, irho)));
     const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
     const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
     const R la = rsqrt_(rmax(R(0), bnx2));
@@ -356,7 +382,9 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
 
 // Row m of L after normalisation and sign flip (src/ssprk.cpp:662-726,
 // 789-806).
-template <int M, typename R>
+// DM: division mode of the reference-rounding normalisation (div_q; the
+// unscaled mode needs e.nrm_half > 0, which the caller checks).
+template <int M, typename R, int DM = DIV_IEEE>
 __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
     const R gam0 = R(1) - gam;
     const R gam1 = R(0.5) * (gam - R(1));
@@ -429,8 +457,8 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
     for (int c = 0; c < 7; ++c) {
 #if HOWMHD_EXACT
         if constexpr (M == 1 || M == 5) L[c] = R(0.5) * L[c];
-        else if constexpr (M == 3) L[c] = div_by<R>(-gam0 * L[c], cs2, e.nrm_half);
-        else L[c] = div_by<R>(R(0.5) * L[c], cs2, e.nrm_half);
+        else if constexpr (M == 3) L[c] = div_q<DM>(-gam0 * L[c], cs2, e.nrm_half);
+        else L[c] = div_q<DM>(R(0.5) * L[c], cs2, e.nrm_half);
 #else
         if constexpr (M == 1 || M == 5) L[c] = R(0.5) * L[c];
         else if constexpr (M == 3) L[c] = L[c] * e.nrm_ent;

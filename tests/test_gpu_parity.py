@@ -126,6 +126,29 @@ def test_dt_from_fused_cfl_maxima(precision, require_gpu):
         assert gpu.tstep() == ref.tstep(), f"{precision}: dt differs after step {step}"
 
 
+@pytest.mark.parametrize("rho_cell", [-0.25, 1e-300])
+def test_fp64_bitwise_nonpositive_density_cell(rho_cell, require_gpu):
+    """One cell of the MHD blast given density -0.25 (the sweep's shared
+    reciprocal 1/rho is then negative, and its tile goes to the IEEE-division
+    launch) or 1e-300, with floors rhomin = pgmin = 1e-3: still bit for bit."""
+    run, q0, p = _setup("mhd_blast3d", dict(nx=16, ny=16, nz=16, rhomin=1e-3, pgmin=1e-3))
+    q0 = q0.copy()
+    H.active(q0)[0, 5, 6, 7] = rho_cell
+    ref = O.RefSolver(p)
+    gpu = H.GpuSolver(p)
+    for s in (ref, gpu):
+        s.upload_q0(q0)
+        s.init_faces()
+    for step in range(2):
+        dt = ref.tstep()
+        assert gpu.tstep() == dt, f"dt differs at step {step}"
+        ref.ssprk(dt)
+        gpu.ssprk(dt)
+    assert np.all(np.isfinite(H.active(ref.q0())))
+    assert _bitwise_mismatch(gpu.q0(), ref.q0()) == 0
+    assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
+
+
 def test_fp64_bitwise_bench_workload(require_gpu):
     """The benchmark workload itself, MHD blast at 256^3: one full step
     (tstep + SSPRK(5,4)) of the GPU path equals the restatement bit for bit
