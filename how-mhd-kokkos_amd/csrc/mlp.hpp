@@ -131,7 +131,12 @@ __device__ __forceinline__ void mlp_features(const float* rho, const float* p, c
             hi = fmaxf(hi, src[g][k]);
         }
         const float span = hi - lo;
-        const float inv = span > MLP_FLAT * fmaxf(fabsf(hi), fabsf(lo)) + 1e-30f ? 1.0f / span : 0.0f;
+#if HOWMHD_EXACT
+        const float rspan = 1.0f / span;
+#else
+        const float rspan = __builtin_amdgcn_rcpf(span);  // FMA forms: hardware reciprocal (<= 1 ulp)
+#endif
+        const float inv = span > MLP_FLAT * fmaxf(fabsf(hi), fabsf(lo)) + 1e-30f ? rspan : 0.0f;
 #pragma unroll
         for (int k = 0; k < 6; ++k) f[6 * g + k] = (src[g][k] - lo) * inv;
     }
