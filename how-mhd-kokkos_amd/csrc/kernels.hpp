@@ -77,12 +77,8 @@ template <> struct SweepTile<2> {
     static constexpr int LANES = HOWMHD_YZ_LANES, ROWS = 256 / HOWMHD_YZ_LANES, CELLS = LANES * (ROWS + 5);
 };
 
-// Waves whose characteristic projections share one pass over the stencil in
-// LDS, and the minimum number of wavefronts per SIMD the sweep kernel is
-// compiled for (caps its registers at 512 / that number).
-#ifndef HOWMHD_WAVE_GROUP
-#define HOWMHD_WAVE_GROUP 1
-#endif
+// The minimum number of wavefronts per SIMD the FP64 sweeps are compiled for
+// (caps their registers at 512 / that number).
 #ifndef HOWMHD_SWEEP_WAVES
 #define HOWMHD_SWEEP_WAVES 2
 #endif
@@ -163,17 +159,6 @@ __device__ __forceinline__ long long sample_index(const Geom& g, int t, int x, i
     if constexpr (DIR == 0) return ((long long)z * g.n[1] + y) * nint + u;
     else if constexpr (DIR == 1) return ((long long)z * nint + u) * g.n[0] + x;
     else return (u * g.n[1] + y) * g.n[0] + x;
-}
-
-template <int M, typename R>
-__device__ __forceinline__ R wave_speed(R vn, R lf, R la, R ls) {
-    if constexpr (M == 0) return vn - lf;
-    else if constexpr (M == 1) return vn - la;
-    else if constexpr (M == 2) return vn - ls;
-    else if constexpr (M == 3) return vn;
-    else if constexpr (M == 4) return vn + ls;
-    else if constexpr (M == 5) return vn + la;
-    else return vn + lf;
 }
 
 template <int B, int E, typename F>

@@ -146,11 +146,6 @@ enum CellField {
 // primit_x_all + eigenst_x_all centre for one cell (src/ssprk.cpp:470-593).
 // Input: conserved in sweep order, FP64.  The primitive conversion, the
 // floors and E -> p stay in FP64; the outputs are rounded to R.
-template <typename R>
-struct CellOut {
-    R f[C_NFIELD];
-};
-
 // DM: division mode of the reference-rounding form; in the unscaled mode
 // *bad is raised for a cell whose shared reciprocals are not positive (it is
 // then recomputed with div_by).

@@ -39,12 +39,6 @@ __device__ __forceinline__ uint32_t f2bf2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 
-// four values as one bf16x4 (64-bit LDS store)
-__device__ __forceinline__ bf16x4 f2bf4(float a, float b, float c, float d) {
-    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 w = {f2bf2(a, b), f2bf2(c, d)};
-    return __builtin_bit_cast(bf16x4, w);
-}
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
 // Device form of the weights for the sweep's network: per-lane A fragments of
