@@ -214,7 +214,9 @@ def main():
     if args.indicator:
         P, thr = H.load_indicator(WEIGHTS)
         gpu.set_indicator(P, thr, enable=1)
-    gpu.upload_q0(q0)
+    t_up = time.perf_counter()
+    gpu.upload_q0(q0)            # host (reference layout, ghosts) -> HBM, synchronous
+    t_up = time.perf_counter() - t_up
     gpu.init_faces()
     del q0
     for _ in range(args.warmup):
@@ -229,6 +231,11 @@ def main():
     el = time.perf_counter() - t0
     zones = n ** 3
     value = zones * args.steps / el
+    t_dn = time.perf_counter()
+    qd = gpu.q0()                # HBM -> host (reference layout, ghosts filled), synchronous
+    t_dn = time.perf_counter() - t_dn
+    host_bytes = qd.nbytes
+    del qd
 
     if args.copy_probe_mib > 0:
         nel = args.copy_probe_mib * 1024 * 1024 // 8
@@ -303,6 +310,13 @@ def main():
         "kernels": kernels,
         "whole_step": whole,
         "divb_ct_max": d.max_divb_ct,
+        # the C-ABI hands over host buffers in the reference layout: one upload
+        # of q(0) before and one download after the timed steps, host layout
+        # transposition included (not part of `value`)
+        "host_boundary": {
+            "upload_s": round(t_up, 4), "download_s": round(t_dn, 4), "bytes_each": host_bytes,
+            "pcie_inclusive_zone_cycles_per_s": round(zones * args.steps / (el + t_up + t_dn), 1),
+        },
     }
     if not args.no_forms and args.precision == "fp64" and not args.indicator:
         forms = {"fp64fast": form_leg(H, torch, run, args, value, "fp64fast"),
