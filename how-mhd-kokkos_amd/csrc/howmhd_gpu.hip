@@ -1,11 +1,3 @@
-// C-ABI of the HIP time step (declared in include/howmhd_gpu.h).
-//
-// One context owns the device state of a run: five stage buffers of q and of
-// the faces (stage 5 reuses the buffer of stage 1, which is dead after stage
-// 2; the step ends by swapping it with stage 0 instead of the reference's
-// copy_stage(q,0,5)), the three directional interface-flux arrays, the six CT
-// flux arrays and the six EMF arrays.  All work is queued on the context's
-// stream; tstep and downloads synchronise.
 // C-ABI of the HIP time step (declared in include/howmhd_gpu.h and
 // include/howmhd_mlp.h).
 //
@@ -100,39 +92,58 @@ uint16_t bf16_bits(float f) {
     return (uint16_t)(u >> 16);
 }
 
-// reference layout (m, i, j, k) with ghosts <-> device layout (m, z, y, x) active
-void ref_to_dev(const howmhd_gpu_ctx* c, const double* ref, double* dev, int nvar) {
-    const int nx = c->g.n[0], ny = c->g.n[1], nz = c->g.n[2];
-    const size_t n6y = (size_t)ny + 6, n6z = (size_t)nz + 6, n6 = ((size_t)nx + 6) * n6y * n6z;
-    for (int m = 0; m < nvar; ++m)
-        for (int x = 0; x < nx; ++x)
-            for (int y = 0; y < ny; ++y) {
-                const double* src = ref + (size_t)m * n6 + (((size_t)x + 3) * n6y + (size_t)y + 3) * n6z + 3;
-                double* dst = dev + (size_t)m * c->ncell + (size_t)y * nx + x;
-                for (int z = 0; z < nz; ++z) dst[(size_t)z * nx * ny] = src[z];
-            }
-}
-
 int host_map(int a, int n, int per) {
     if (n == 1) return 0;
     if (per) return ((a % n) + n) % n;
     return a < 0 ? 0 : (a >= n ? n - 1 : a);
 }
 
-void dev_to_ref(const howmhd_gpu_ctx* c, const double* dev, double* ref, int nvar) {
-    const int nx = c->g.n[0], ny = c->g.n[1], nz = c->g.n[2];
-    const size_t n6x = (size_t)nx + 6, n6y = (size_t)ny + 6, n6z = (size_t)nz + 6, n6 = n6x * n6y * n6z;
-    std::vector<int> mx(n6x), my(n6y), mz(n6z);
-    for (size_t i = 0; i < n6x; ++i) mx[i] = host_map((int)i - 3, nx, c->g.per[0]);
-    for (size_t j = 0; j < n6y; ++j) my[j] = host_map((int)j - 3, ny, c->g.per[1]);
-    for (size_t k = 0; k < n6z; ++k) mz[k] = host_map((int)k - 3, nz, c->g.per[2]);
-    for (int m = 0; m < nvar; ++m)
-        for (size_t i = 0; i < n6x; ++i)
-            for (size_t j = 0; j < n6y; ++j) {
-                double* dst = ref + (size_t)m * n6 + (i * n6y + j) * n6z;
-                const double* src = dev + (size_t)m * c->ncell + (size_t)my[j] * nx + mx[i];
-                for (size_t k = 0; k < n6z; ++k) dst[k] = src[(size_t)mz[k] * nx * ny];
-            }
+// Transfers between the caller's arrays in the reference layout
+// (m, i, j, k), three ghost layers per side, and the device layout
+// (m, z, y, x) of active cells.  One variable at a time goes through a device
+// buffer of one variable's reference-layout size, allocated for the call: the
+// copy reads or writes the caller's (pageable) memory directly and the
+// transpose runs on the device (k_ref_to_dev, k_dev_to_ref).
+struct StageBuf {
+    double* p = nullptr;
+    ~StageBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+size_t ref_size(const howmhd_gpu_ctx* c) {
+    return ((size_t)c->g.n[0] + 6) * ((size_t)c->g.n[1] + 6) * ((size_t)c->g.n[2] + 6);
+}
+
+unsigned lay_planes(int n) { return (unsigned)std::min(n, 65535); }
+
+int upload_ref(howmhd_gpu_ctx* c, const double* ref, double* dev, int nvar) {
+    const size_t n6 = ref_size(c);
+    StageBuf st;
+    HCHECK(c, hipMalloc(&st.p, n6 * sizeof(double)));
+    const dim3 grid(blocks_for(c->g.n[2], LAY_T), blocks_for(c->g.n[0], LAY_T), lay_planes(c->g.n[1]));
+    for (int m = 0; m < nvar; ++m) {
+        HCHECK(c, hipMemcpyAsync(st.p, ref + (size_t)m * n6, n6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_ref_to_dev, grid, dim3(LAY_T, 8), 0, c->stream, (const double*)st.p,
+                           dev + (size_t)m * c->ncell, c->g);
+        HCHECK(c, hipGetLastError());
+    }
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    return HOWMHD_OK;
+}
+
+int download_ref(howmhd_gpu_ctx* c, const double* dev, double* ref, int nvar) {
+    const size_t n6 = ref_size(c);
+    StageBuf st;
+    HCHECK(c, hipMalloc(&st.p, n6 * sizeof(double)));
+    const dim3 grid(blocks_for(c->g.n[2] + 6, LAY_T), blocks_for(c->g.n[0] + 6, LAY_T), lay_planes(c->g.n[1] + 6));
+    for (int m = 0; m < nvar; ++m) {
+        hipLaunchKernelGGL(k_dev_to_ref, grid, dim3(LAY_T, 8), 0, c->stream, dev + (size_t)m * c->ncell, st.p, c->g);
+        HCHECK(c, hipGetLastError());
+        HCHECK(c, hipMemcpyAsync(ref + (size_t)m * n6, st.p, n6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    return HOWMHD_OK;
 }
 
 }  // namespace
@@ -267,11 +278,8 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
 
 int howmhd_gpu_upload_q0(howmhd_gpu_ctx* c, const double* q0) {
     if (!c || !q0) return HOWMHD_EINVAL;
-    std::vector<double> h(8 * c->ncell);
-    ref_to_dev(c, q0, h.data(), 8);
-    HCHECK(c, hipMemcpyAsync(c->qbuf[c->qid[0]], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                             c->stream));
-    HCHECK(c, hipStreamSynchronize(c->stream));
+    const int rc = upload_ref(c, q0, c->qbuf[c->qid[0]], 8);
+    if (rc) return rc;
     c->faces_ready = false;
     c->cfl_fresh = false;
     return HOWMHD_OK;
@@ -289,11 +297,8 @@ int howmhd_gpu_init_faces(howmhd_gpu_ctx* c) {
 
 int howmhd_gpu_upload_faces(howmhd_gpu_ctx* c, const double* b) {
     if (!c || !b) return HOWMHD_EINVAL;
-    std::vector<double> h(3 * c->ncell);
-    ref_to_dev(c, b, h.data(), 3);
-    HCHECK(c, hipMemcpyAsync(c->fbuf[c->qid[0]], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                             c->stream));
-    HCHECK(c, hipStreamSynchronize(c->stream));
+    const int rc = upload_ref(c, b, c->fbuf[c->qid[0]], 3);
+    if (rc) return rc;
     c->faces_ready = true;
     return HOWMHD_OK;
 }
@@ -384,22 +389,12 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
 
 int howmhd_gpu_download_q0(howmhd_gpu_ctx* c, double* q0) {
     if (!c || !q0) return HOWMHD_EINVAL;
-    std::vector<double> h(8 * c->ncell);
-    HCHECK(c, hipMemcpyAsync(h.data(), c->qbuf[c->qid[0]], h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                             c->stream));
-    HCHECK(c, hipStreamSynchronize(c->stream));
-    dev_to_ref(c, h.data(), q0, 8);
-    return HOWMHD_OK;
+    return download_ref(c, c->qbuf[c->qid[0]], q0, 8);
 }
 
 int howmhd_gpu_download_faces(howmhd_gpu_ctx* c, double* b) {
     if (!c || !b) return HOWMHD_EINVAL;
-    std::vector<double> h(3 * c->ncell);
-    HCHECK(c, hipMemcpyAsync(h.data(), c->fbuf[c->qid[0]], h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                             c->stream));
-    HCHECK(c, hipStreamSynchronize(c->stream));
-    dev_to_ref(c, h.data(), b, 3);
-    return HOWMHD_OK;
+    return download_ref(c, c->fbuf[c->qid[0]], b, 3);
 }
 
 int howmhd_gpu_diag(howmhd_gpu_ctx* c, howmhd_diag* d) {

@@ -1,6 +1,7 @@
-// Kernels of the per-step calls (tstep and the face initialisation), built
-// once, in the plain FP64 unit (-ffp-contract=off): the CFL maximum and dt
-// are the reference's bit for bit for every form.
+// Kernels of the per-step calls (tstep and the face initialisation) and of
+// the host transfers, built once, in the plain FP64 unit
+// (-ffp-contract=off): the CFL maximum and dt are the reference's bit for bit
+// for every form.
 #pragma once
 
 #include "kernels.hpp"
@@ -90,6 +91,65 @@ __global__ __launch_bounds__(256) void k_init_faces(const double* q, double* b, 
                       By[lin(g, x, bmap(y + 2, ny, py), z)]) / 16.0;
     b[2 * g.ncell + i] = (-Bz[lin(g, x, y, bmap(z - 1, nz, pz))] + 9.0 * Bz[i] + 9.0 * Bz[lin(g, x, y, bmap(z + 1, nz, pz))] -
                           Bz[lin(g, x, y, bmap(z + 2, nz, pz))]) / 16.0;
+}
+
+// ------------------------------------------- host layout <-> device layout
+// One variable in the reference's host layout: (nx+6) x (ny+6) x (nz+6)
+// values, three ghost layers per side, z fastest (the q(m, i, j, k) view of
+// src/bound.cpp:19-95 for one m); on the device: active cells only, x
+// fastest.  Both directions transpose (x, z) tiles of 32 x 32 through LDS at
+// fixed y, so that global reads and writes are contiguous along a wavefront
+// on both sides.  Block 32 x 8; grid (z tiles, x tiles, y planes), the
+// planes strided by gridDim.z.
+constexpr int LAY_T = 32;
+
+// host -> device: active cells of the reference layout
+__global__ __launch_bounds__(256) void k_ref_to_dev(const double* __restrict__ ref, double* __restrict__ dev, Geom g) {
+    __shared__ double t[LAY_T][LAY_T + 1];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const size_t n6y = (size_t)ny + 6, n6z = (size_t)nz + 6;
+    const int z0 = blockIdx.x * LAY_T, x0 = blockIdx.y * LAY_T;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int y = blockIdx.z; y < ny; y += gridDim.z) {
+#pragma unroll
+        for (int r = 0; r < LAY_T; r += 8) {
+            const int x = x0 + ty + r, z = z0 + tx;
+            if (x < nx && z < nz) t[ty + r][tx] = ref[(((size_t)x + 3) * n6y + (size_t)y + 3) * n6z + (size_t)z + 3];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < LAY_T; r += 8) {
+            const int z = z0 + ty + r, x = x0 + tx;
+            if (x < nx && z < nz) dev[((size_t)z * ny + y) * nx + x] = t[tx][ty + r];
+        }
+        __syncthreads();
+    }
+}
+
+// device -> host: every cell of the reference layout, the ghost layers filled
+// with the active cells the boundary conditions map them to (src/bound.cpp)
+__global__ __launch_bounds__(256) void k_dev_to_ref(const double* __restrict__ dev, double* __restrict__ ref, Geom g) {
+    __shared__ double t[LAY_T][LAY_T + 1];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int n6x = nx + 6, n6y = ny + 6, n6z = nz + 6;
+    const int k0 = blockIdx.x * LAY_T, i0 = blockIdx.y * LAY_T;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int j = blockIdx.z; j < n6y; j += gridDim.z) {
+        const size_t yo = (size_t)bmap(j - 3, ny, g.per[1]) * nx;
+#pragma unroll
+        for (int r = 0; r < LAY_T; r += 8) {
+            const int k = k0 + ty + r, i = i0 + tx;
+            if (i < n6x && k < n6z)
+                t[ty + r][tx] = dev[(size_t)bmap(k - 3, nz, g.per[2]) * ny * nx + yo + bmap(i - 3, nx, g.per[0])];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < LAY_T; r += 8) {
+            const int i = i0 + ty + r, k = k0 + tx;
+            if (i < n6x && k < n6z) ref[((size_t)i * n6y + j) * n6z + k] = t[tx][ty + r];
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace howmhd
