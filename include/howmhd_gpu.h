@@ -102,7 +102,11 @@ HOWMHD_API int howmhd_gpu_destroy(howmhd_gpu_ctx* ctx);
 
 /* Copies stage-0 q from the host (reference layout, ghosts ignored).  The
  * reference's problem initialisers write q(0,...) in place
- * (src/problem.cpp:39-59); this is the device hand-over of that array. */
+ * (src/problem.cpp:39-59); this is the device hand-over of that array.
+ * Host transfers (this call, upload_faces and the downloads) read or write
+ * the caller's memory directly, one variable at a time, through a device
+ * buffer of one variable's reference-layout size that the call allocates
+ * and frees; the layout change runs on the device.  Synchronises. */
 HOWMHD_API int howmhd_gpu_upload_q0(howmhd_gpu_ctx* ctx, const double* q0);
 
 /* bound(q,0) + initialize_bfield_from_q + bound_bfield(0)
