@@ -667,6 +667,188 @@ __global__ __launch_bounds__(256) void k_emf(SmoothArgs<R> A) {
     }
 }
 
+// k_emf2 and k_emf in one pass for three-dimensional grids (every CT flag
+// set): the corner EMFs O2 of src/fluxct.cpp:809-821 and their smoothing,
+// 828-840, without the O2 round trip through memory.  A block of 512
+// threads holds a 64 x 8 tile of columns (EMF_OX x EMF_OY owned columns and a
+// one-column halo) and marches through EMF_CZ planes.  Per plane z:
+//  * the in-plane CT fluxes of plane z + 1 (gsx, hsx with a 1|2-column halo
+//    for their x stencils, fsy, hsy with a 1|2-row halo) were loaded one
+//    plane ahead and go to LDS; gsz and fsz come from per-column registers
+//    holding planes z .. z + 3;
+//  * each tile position computes O2 at z + 1 into the other half of a
+//    double-buffered LDS tile;
+//  * each owned column smooths O2 at z: in-plane neighbours from LDS, z
+//    neighbours (O2y, O2z at z - 1, z + 1) from its registers.
+// Tile positions are unmapped coordinates; a position's CT values are those
+// of its bmap image.  Periodic images are translations, so a halo position's
+// O2 is the O2 of its image; at an open edge the image of the outside
+// neighbour is the cell itself (bmap clamps), so owned cells there read their
+// own O2 instead of the halo's.  The first two planes of a chunk (O2 at
+// z0 - 1 and z0) come straight from memory through bmap, as in k_emf2.  The
+// arithmetic is k_emf2's and k_emf's, term for term.
+#ifndef HOWMHD_EMF_WX
+#define HOWMHD_EMF_WX 64
+#endif
+#ifndef HOWMHD_EMF_WY
+#define HOWMHD_EMF_WY 8
+#endif
+#ifndef HOWMHD_EMF_CZ
+#define HOWMHD_EMF_CZ 32
+#endif
+constexpr int EMF_WX = HOWMHD_EMF_WX, EMF_WY = HOWMHD_EMF_WY, EMF_CZ = HOWMHD_EMF_CZ;
+constexpr int EMF_OX = EMF_WX - 2, EMF_OY = EMF_WY - 2;
+constexpr int EMF_THREADS = EMF_WX * EMF_WY;
+static_assert(6 * (EMF_WX + EMF_WY) <= EMF_THREADS, "one stencil-halo cell per thread at most");
+
+template <typename R>
+struct EmfMarchArgs {
+    const R *fsy, *fsz, *gsx, *gsz, *hsx, *hsy;
+    R *Ox, *Oy, *Oz;
+    Geom g;
+};
+
+template <typename R, int EX>
+__global__ __launch_bounds__(EMF_THREADS) void k_emf_march(EmfMarchArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
+    constexpr int WX = EMF_WX, WY = EMF_WY;
+    constexpr int NXH = 3 * WY, NYH = 3 * WX;  // stencil halo cells of the x and y arrays
+    __shared__ R sgx[WY][WX + 3], shx[WY][WX + 3];  // column c holds position c - 1 (x stencils)
+    __shared__ R sfy[WY + 3][WX], shy[WY + 3][WX];  // row r holds position r - 1 (y stencils)
+    __shared__ R sO[2][3][WY][WX];                  // O2 of two planes
+    const Geom& g = A.g;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int px = g.per[0], py = g.per[1], pz = g.per[2];
+    const int x0 = blockIdx.x * EMF_OX, y0 = blockIdx.y * EMF_OY;
+    const int zs = blockIdx.z * EMF_CZ, ze = min(zs + EMF_CZ, nz);
+    const int tid = threadIdx.x;
+    const int ex = tid % WX, ey = tid / WX;
+    // image of tile position p (x: column, y: row); positions beyond the
+    // stencil reach of any needed O2 are clamped into bmap's range
+    auto imx = [&](int pos) { const int a = x0 - 1 + pos; return bmap(a > nx + 2 ? nx + 2 : a, nx, px); };
+    auto imy = [&](int pos) { const int a = y0 - 1 + pos; return bmap(a > ny + 2 ? ny + 2 : a, ny, py); };
+    const int xm = imx(ex), ym = imy(ey);
+    const int ax = x0 - 1 + ex, ay = y0 - 1 + ey;
+    const bool own = ex >= 1 && ex <= EMF_OX && ey >= 1 && ey <= EMF_OY && ax < nx && ay < ny;
+    // in-plane neighbour positions of an owned cell (open edges: itself)
+    const int exl = (ax == 0 && !px) ? ex : ex - 1, exr = (ax == nx - 1 && !px) ? ex : ex + 1;
+    const int eyl = (ay == 0 && !py) ? ey : ey - 1, eyr = (ay == ny - 1 && !py) ? ey : ey + 1;
+    // this thread's stencil-halo cell: (array, LDS row, LDS column, image)
+    int hk = -1, hr = 0, hc = 0, hxm = 0, hym = 0;
+    if (tid < 2 * NXH) {             // gsx / hsx: columns at positions -1, WX, WX + 1
+        hk = tid < NXH ? 0 : 1;
+        const int t = tid % NXH, col = t % 3, row = t / 3;
+        const int pos = col == 0 ? -1 : WX - 1 + col;
+        hr = row; hc = pos + 1; hxm = imx(pos); hym = imy(row);
+    } else if (tid < 2 * NXH + 2 * NYH) {  // fsy / hsy: rows at positions -1, WY, WY + 1
+        const int t0 = tid - 2 * NXH;
+        hk = t0 < NYH ? 2 : 3;
+        const int t = t0 % NYH, row = t / WX, col = t % WX;
+        const int pos = row == 0 ? -1 : WY - 1 + row;
+        hr = pos + 1; hc = col; hxm = imx(col); hym = imy(pos);
+    }
+    const R* harr = hk == 0 ? A.gsx : hk == 1 ? A.hsx : hk == 2 ? A.fsy : A.hsy;
+
+    // O2 (k_emf2) of stored cell (x, y, z), straight from memory
+    auto o2 = [&](int x, int y, int z, R& ox2, R& oy2, R& oz2) {
+        const int xm1 = bmap(x - 1, nx, px), xp1 = bmap(x + 1, nx, px), xp2 = bmap(x + 2, nx, px);
+        const int ym1 = bmap(y - 1, ny, py), yp1 = bmap(y + 1, ny, py), yp2 = bmap(y + 2, ny, py);
+        const int zm1 = bmap(z - 1, nz, pz), zp1 = bmap(z + 1, nz, pz), zp2 = bmap(z + 2, nz, pz);
+        const long long i = lin(g, x, y, z);
+        ox2 = i4<R>(A.gsx[lin(g, xm1, y, z)], A.gsx[i], A.gsx[lin(g, xp1, y, z)], A.gsx[lin(g, xp2, y, z)]) -
+              i4<R>(A.fsy[lin(g, x, ym1, z)], A.fsy[i], A.fsy[lin(g, x, yp1, z)], A.fsy[lin(g, x, yp2, z)]);
+        oy2 = i4<R>(A.hsy[lin(g, x, ym1, z)], A.hsy[i], A.hsy[lin(g, x, yp1, z)], A.hsy[lin(g, x, yp2, z)]) -
+              i4<R>(A.gsz[lin(g, x, y, zm1)], A.gsz[i], A.gsz[lin(g, x, y, zp1)], A.gsz[lin(g, x, y, zp2)]);
+        oz2 = i4<R>(A.fsz[lin(g, x, y, zm1)], A.fsz[i], A.fsz[lin(g, x, y, zp1)], A.fsz[lin(g, x, y, zp2)]) -
+              i4<R>(A.hsx[lin(g, xm1, y, z)], A.hsx[i], A.hsx[lin(g, xp1, y, z)], A.hsx[lin(g, xp2, y, z)]);
+    };
+    // loads of plane m (an image plane) for the in-plane stencils
+    R cgx, chx, cfy, chy, chalo = R(0);
+    auto load_plane = [&](int m) {
+        const long long i = lin(g, xm, ym, m);
+        cgx = A.gsx[i]; chx = A.hsx[i]; cfy = A.fsy[i]; chy = A.hsy[i];
+        if (hk >= 0) chalo = harr[lin(g, hxm, hym, m)];
+    };
+
+    // prologue: O2 at z0 - 1 (y, z components) and z0, from memory
+    R ymv, zmv, yc, zc;
+    {
+        R ox, unused;
+        o2(xm, ym, bmap(zs - 1, nz, pz), unused, ymv, zmv);
+        o2(xm, ym, zs, ox, yc, zc);
+        sO[zs & 1][0][ey][ex] = ox;
+        sO[zs & 1][1][ey][ex] = yc;
+        sO[zs & 1][2][ey][ex] = zc;
+    }
+    // gsz, fsz of this column at planes z .. z + 3 for O2 at z + 1 (the
+    // first entry is shifted out before use)
+    const long long col = lin(g, xm, ym, 0);
+    const long long plane = (long long)nx * ny;
+    R gz[4], fz[4];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        const int m = bmap(zs + k - 1, nz, pz);
+        gz[k] = A.gsz[col + m * plane];
+        fz[k] = A.fsz[col + m * plane];
+    }
+    gz[0] = fz[0] = R(0);
+    R gzn = A.gsz[col + bmap(zs + 3, nz, pz) * plane], fzn = A.fsz[col + bmap(zs + 3, nz, pz) * plane];
+    load_plane(bmap(zs + 1, nz, pz));
+
+    for (int z = zs; z < ze; ++z) {
+        // in-plane fluxes of plane z + 1 to LDS; column window z .. z + 3
+        sgx[ey][ex + 1] = cgx;
+        shx[ey][ex + 1] = chx;
+        sfy[ey + 1][ex] = cfy;
+        shy[ey + 1][ex] = chy;
+        if (hk == 0) sgx[hr][hc] = chalo;
+        else if (hk == 1) shx[hr][hc] = chalo;
+        else if (hk == 2) sfy[hr][hc] = chalo;
+        else if (hk == 3) shy[hr][hc] = chalo;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { gz[k] = gz[k + 1]; fz[k] = fz[k + 1]; }
+        gz[3] = gzn; fz[3] = fzn;
+        // loads of the next plane, in flight across the barriers below
+        if (z + 1 < ze) {
+            load_plane(bmap(z + 2, nz, pz));
+            const int m = bmap(z + 4, nz, pz);
+            gzn = A.gsz[col + m * plane];
+            fzn = A.fsz[col + m * plane];
+        }
+        __syncthreads();
+        // O2 at z + 1 at this tile position (open top edge: the image of
+        // plane nz is plane nz - 1, whose O2 is this iteration's)
+        R yn, zn;
+        const int cb = (z + 1) & 1;
+        if (z + 1 == nz && !pz) {
+            yn = yc; zn = zc;
+        } else {
+            const R ox2 = i4<R>(sgx[ey][ex], sgx[ey][ex + 1], sgx[ey][ex + 2], sgx[ey][ex + 3]) -
+                          i4<R>(sfy[ey][ex], sfy[ey + 1][ex], sfy[ey + 2][ex], sfy[ey + 3][ex]);
+            yn = i4<R>(shy[ey][ex], shy[ey + 1][ex], shy[ey + 2][ex], shy[ey + 3][ex]) -
+                 i4<R>(gz[0], gz[1], gz[2], gz[3]);
+            zn = i4<R>(fz[0], fz[1], fz[2], fz[3]) -
+                 i4<R>(shx[ey][ex], shx[ey][ex + 1], shx[ey][ex + 2], shx[ey][ex + 3]);
+            sO[cb][0][ey][ex] = ox2;
+            sO[cb][1][ey][ex] = yn;
+            sO[cb][2][ey][ex] = zn;
+        }
+        // smoothing of O2 at z (k_emf)
+        if (own) {
+            const int c = z & 1;
+            const long long i = lin(g, xm, ym, z);
+            const R ox = sO[c][0][ey][ex], oy = yc, oz = zc;
+            A.Ox[i] = ox + lap24<R>(sO[c][0][ey][exl], ox, sO[c][0][ey][exr]) +
+                      lap24<R>(sO[c][0][eyl][ex], ox, sO[c][0][eyr][ex]);
+            A.Oy[i] = oy + lap24<R>(sO[c][1][eyl][ex], oy, sO[c][1][eyr][ex]) + lap24<R>(ymv, oy, yn);
+            A.Oz[i] = oz + lap24<R>(sO[c][2][ey][exl], oz, sO[c][2][ey][exr]) + lap24<R>(zmv, oz, zn);
+        }
+        __syncthreads();
+        ymv = yc; zmv = zc;
+        yc = yn; zc = zn;
+    }
+}
+
 struct StageCoef {
     double k1, k3, c4, c5;  // c4 = k4/k1, c5 = k5/k1 (stage 5 only)
     double dtdx, dtdy, dtdz;

@@ -115,7 +115,20 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     if (rc) return rc;
 
     const unsigned nb = blocks_for(c->ncell, 256);
-    if (c->ctf.active) {
+    if (c->ctf.use_x && c->ctf.use_y && c->ctf.use_z) {
+        EmfMarchArgs<R> e;
+        e.fsy = (const R*)c->ct[0]; e.fsz = (const R*)c->ct[1];
+        e.gsx = (const R*)c->ct[2]; e.gsz = (const R*)c->ct[3];
+        e.hsx = (const R*)c->ct[4]; e.hsy = (const R*)c->ct[5];
+        e.Ox = (R*)c->O[0]; e.Oy = (R*)c->O[1]; e.Oz = (R*)c->O[2];
+        e.g = c->g;
+        const dim3 grid(blocks_for(c->g.n[0], EMF_OX), blocks_for(c->g.n[1], EMF_OY), blocks_for(c->g.n[2], EMF_CZ));
+        {
+            TimedLaunch tl(c, HOWMHD_K_EMF);
+            hipLaunchKernelGGL((k_emf_march<R, EX>), grid, dim3(EMF_THREADS), 0, c->stream, e);
+        }
+        HCHECK(c, hipGetLastError());
+    } else if (c->ctf.active) {
         EmfArgs<R> e;
         e.fsy = (const R*)c->ct[0]; e.fsz = (const R*)c->ct[1];
         e.gsx = (const R*)c->ct[2]; e.gsz = (const R*)c->ct[3];
