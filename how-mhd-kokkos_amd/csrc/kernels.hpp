@@ -11,7 +11,9 @@
 //   k_sweep<X|Y|Z>  fused primit + eigensystem + characteristic LLF-WENO5:
 //                   interface fluxes dF (7 per interface) and CT flux terms
 //                   (src/ssprk.cpp:458-997, 1000-1272)
-//   k_emf2, k_emf   corner EMF build and smoothing (src/fluxct.cpp:809-840)
+//   k_emf_march     corner EMF build and smoothing (src/fluxct.cpp:809-840)
+//                   in one z-marching pass on 3-D grids; k_emf2, k_emf as two
+//                   passes on 1-D and 2-D grids
 //   k_faces         face-B RK combination + 4th-order staggered curl
 //                   (src/fluxct.cpp:848-877, 111-135)
 //   k_combine       stage combination k1*q_init + k3*q_prev + dX + dY + dZ,

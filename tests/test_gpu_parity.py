@@ -32,6 +32,11 @@ CASES = {
     "cpaw": ("cpaw3d", dict(nx=24, ny=12, nz=12, xsize=3.0, ysize=1.5, zsize=1.5), 2),
     "xz2d": ("mpi_smooth3d", dict(nx=20, ny=1, nz=18), 3),
     "floors": ("blast3d", dict(nx=16, ny=16, nz=8, rhomin=0.9999, pgmin=5.0), 2),
+    # every boundary open (the 3-D EMF kernel's open edges in x, y and z)
+    "blast_open3": ("mhd_blast3d", dict(nx=14, ny=13, nz=12, x1bc="open", x2bc="open", x3bc="open"), 2),
+    # two z chunks of the 3-D EMF kernel, the second one plane deep; ragged
+    # x/y tiles
+    "tall33": ("mpi_smooth3d", dict(nx=10, ny=7, nz=33), 2),
 }
 
 
