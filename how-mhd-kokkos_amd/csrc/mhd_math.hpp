@@ -28,10 +28,14 @@ __device__ __forceinline__ R rmax(R a, R b) { return fmax(a, b); }
 template <>
 __device__ __forceinline__ float rmax<float>(float a, float b) { return fmaxf(a, b); }
 
+// Square root: FP64 the IEEE square root; FP32 (the FP32 form only) the
+// hardware v_sqrt_f32, within 1 ulp, instead of the correctly rounded
+// sequence (range scaling, a +-1 ulp correction and a class select, 15
+// instructions).
 template <typename R>
 __device__ __forceinline__ R rsqrt_(R x) { return sqrt(x); }
 template <>
-__device__ __forceinline__ float rsqrt_<float>(float x) { return sqrtf(x); }
+__device__ __forceinline__ float rsqrt_<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <typename R>
 __device__ __forceinline__ R rabs(R x) { return fabs(x); }
@@ -69,6 +73,16 @@ __device__ __forceinline__ double div_with(double a, double b, double r) {
     return __builtin_fma(__builtin_fma(-b, q, a), r, q);
 }
 
+
+// FP64 1/x from v_rcp_f64 and two Newton-Raphson steps, within an ulp of the
+// quotient for normal x (no range scaling; x = 0 gives NaN, not inf).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
 
 // 1/x in the FMA forms: FP32 takes the hardware reciprocal (v_rcp_f32,
 // within 1 ulp) instead of the IEEE division sequence; FP64 divides.
@@ -164,7 +178,9 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const double vy = div_q<DM>(My, DD, iDD);
     const double vz = div_q<DM>(Mz, DD, iDD);
 #else
-    double irho = 1.0 / rho;
+    // FP32 form: 1/rho from the hardware reciprocal and two Newton steps
+    // (within an ulp of FP64; the quantities leave this function in FP32)
+    double irho = (sizeof(R) == 4) ? rcp_nr(rho) : 1.0 / rho;
     const double vx = Mx * irho;
     const double vy = My * irho;
     const double vz = Mz * irho;
@@ -179,7 +195,7 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
         EE = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
         Dq = rho;
 #if !HOWMHD_EXACT
-        irho = 1.0 / rho;
+        irho = (sizeof(R) == 4) ? rcp_nr(rho) : 1.0 / rho;
 #endif
     }
 #if HOWMHD_EXACT
