@@ -15,7 +15,8 @@
 //                   in one z-marching pass on 3-D grids; k_emf2, k_emf as two
 //                   passes on 1-D and 2-D grids
 //   k_faces         face-B RK combination + 4th-order staggered curl
-//                   (src/fluxct.cpp:848-877, 111-135)
+//                   (src/fluxct.cpp:848-877, 111-135); k_faces_march, the
+//                   same marching through z on 3-D grids
 //   k_combine       stage combination k1*q_init + k3*q_prev + dX + dY + dZ,
 //                   face->centre B (src/fluxct.cpp:882-897) and the floors of
 //                   prot (src/prot.cpp:10-51)
@@ -966,6 +967,138 @@ __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
     A.bo[i] = bx;
     A.bo[g.ncell + i] = by;
     A.bo[2 * g.ncell + i] = bz;
+}
+
+// k_faces for three-dimensional grids (every CT flag set), marching a
+// FACE_TX x FACE_TY column tile through FACE_CZ planes: the in-plane EMF
+// tiles of each plane (the LDS tiles of k_faces) and the face values of the
+// next plane are loaded one plane ahead, and each column keeps Oy and Oz of
+// planes z - 3 .. z + 2 (images through bmap) in registers for the z
+// differences.  The arithmetic is k_faces', term for term.
+constexpr int FACE_CZ = 32;
+// 3 wavefronts per SIMD (136 VGPRs, no scratch in FP64; 4 spills 4 values and
+// runs 4% slower)
+#ifndef HOWMHD_FACE_WAVES
+#define HOWMHD_FACE_WAVES 3
+#endif
+template <typename R, int EX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOWMHD_FACE_WAVES))) void k_faces_march(FaceArgs<R> A) {
+    static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
+    constexpr int HX = FACE_TX + 5, HY = FACE_TY + 5;
+    constexpr int NOX = HY * HX, NOY = HY * FACE_TX, NOZ = FACE_TY * HX, NF = NOX + NOY + NOZ;
+    constexpr int NFT = (NF + 255) / 256;  // tile values per thread
+    // sOx[HY][HX]: rows y0-3 .. y0+TY+1, columns x0-3 .. x0+TX+1; then
+    // sOy[HY][TX] (rows y0-3 ..), sOz[TY][HX] (columns x0-3 ..)
+    __shared__ R sbuf[NF];
+    const R(*sOx)[HX] = reinterpret_cast<const R(*)[HX]>(sbuf);
+    const R(*sOy)[FACE_TX] = reinterpret_cast<const R(*)[FACE_TX]>(sbuf + NOX);
+    const R(*sOz)[HX] = reinterpret_cast<const R(*)[HX]>(sbuf + NOX + NOY);
+    const Geom& g = A.g;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int px = g.per[0], py = g.per[1], pz = g.per[2];
+    const int x0 = blockIdx.x * FACE_TX, y0 = blockIdx.y * FACE_TY;
+    const int zs = blockIdx.z * FACE_CZ, ze = min(zs + FACE_CZ, nz);
+    const int tid = threadIdx.x;
+    const long long plane = (long long)nx * ny;
+    auto hx = [&](int c) { const int a = x0 - 3 + c; return bmap(a > nx + 2 ? nx + 2 : a, nx, px); };
+    auto hy = [&](int r) { const int a = y0 - 3 + r; return bmap(a > ny + 2 ? ny + 2 : a, ny, py); };
+    // this thread's tile values: source column (array and in-plane offset)
+    const R* src[NFT];
+#pragma unroll
+    for (int k = 0; k < NFT; ++k) {
+        const int e = tid + 256 * k;
+        src[k] = nullptr;
+        if (e < NOX) {
+            src[k] = A.Ox + (long long)hy(e / HX) * nx + hx(e % HX);
+        } else if (e < NOX + NOY) {
+            const int t = e - NOX, c = t % FACE_TX, xx = x0 + c < nx ? x0 + c : nx - 1;
+            src[k] = A.Oy + (long long)hy(t / FACE_TX) * nx + xx;
+        } else if (e < NF) {
+            const int t = e - NOX - NOY, r = t / HX, yy = y0 + r < ny ? y0 + r : ny - 1;
+            src[k] = A.Oz + (long long)yy * nx + hx(t % HX);
+        }
+    }
+    const int tx = tid % FACE_TX, ty = tid / FACE_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < nx && y < ny;
+    const long long col = (long long)(valid ? y : 0) * nx + (valid ? x : 0);
+    const StageCoef& s = A.s;
+    R tv[NFT];
+    auto load_tile = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < NFT; ++k)
+            if (src[k]) tv[k] = src[k][(long long)z * plane];
+    };
+    // face values of a plane: b0 (+ b2, b3 at stage 5) and the previous stage
+    double fb0[3], fb2[3], fb3[3], fbp[3];
+    auto load_faces = [&](int z) {
+        const long long i = col + (long long)z * plane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long o = (long long)c * g.ncell + i;
+            fb0[c] = A.b0[o];
+            fbp[c] = A.bp[o];
+            if (s.stage5) { fb2[c] = A.b2[o]; fb3[c] = A.b3[o]; }
+            else { fb2[c] = 0.0; fb3[c] = 0.0; }
+        }
+    };
+    R wy[6], wz[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const long long o = col + (long long)bmap(zs - 3 + k, nz, pz) * plane;
+        wy[k] = A.Oy[o];
+        wz[k] = A.Oz[o];
+    }
+    load_tile(zs);
+    load_faces(zs);
+    for (int z = zs; z < ze; ++z) {
+#pragma unroll
+        for (int k = 0; k < NFT; ++k)
+            if (src[k]) sbuf[tid + 256 * k] = tv[k];
+        double binit[3], bp[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            binit[c] = s.stage5 ? fb0[c] + s.c4 * fb2[c] + s.c5 * fb3[c] : fb0[c];
+            bp[c] = fbp[c];
+        }
+        R ny_ = R(0), nz_ = R(0);
+        if (z + 1 < ze) {  // next plane, in flight across the barriers below
+            load_tile(z + 1);
+            load_faces(z + 1);
+            const long long o = col + (long long)bmap(z + 3, nz, pz) * plane;
+            ny_ = A.Oy[o];
+            nz_ = A.Oz[o];
+        }
+        __syncthreads();
+        if (valid) {
+            const long long i = col + (long long)z * plane;
+            auto col6 = [&](const R (*t)[HX], int c) -> double {
+                return d4((double)t[ty][c], (double)t[ty + 1][c], (double)t[ty + 2][c], (double)t[ty + 3][c],
+                          (double)t[ty + 4][c], (double)t[ty + 5][c]);
+            };
+            auto row6 = [&](const R* r) -> double {
+                return d4((double)r[tx], (double)r[tx + 1], (double)r[tx + 2], (double)r[tx + 3], (double)r[tx + 4],
+                          (double)r[tx + 5]);
+            };
+            const double dzOy = d4((double)wy[0], (double)wy[1], (double)wy[2], (double)wy[3], (double)wy[4],
+                                   (double)wy[5]);
+            const double dzOz = d4((double)wz[0], (double)wz[1], (double)wz[2], (double)wz[3], (double)wz[4],
+                                   (double)wz[5]);
+            const double bx = s.k1 * binit[0] + s.k3 * bp[0] - s.dtdy * col6(sOx, tx + 3) + s.dtdz * dzOz;
+            const double by = s.k1 * binit[1] + s.k3 * bp[1] - s.dtdz * dzOy + s.dtdx * row6(sOx[ty + 3]);
+            const double dyOy = d4((double)sOy[ty][tx], (double)sOy[ty + 1][tx], (double)sOy[ty + 2][tx],
+                                   (double)sOy[ty + 3][tx], (double)sOy[ty + 4][tx], (double)sOy[ty + 5][tx]);
+            const double bz = s.k1 * binit[2] + s.k3 * bp[2] - s.dtdx * row6(sOz[ty]) + s.dtdy * dyOy;
+            A.bo[i] = bx;
+            A.bo[g.ncell + i] = by;
+            A.bo[2 * g.ncell + i] = bz;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 5; ++k) { wy[k] = wy[k + 1]; wz[k] = wz[k + 1]; }
+        wy[5] = ny_;
+        wz[5] = nz_;
+    }
 }
 
 // CFL speeds of one cell (src/tstep.cpp:28-83): the running maxima of

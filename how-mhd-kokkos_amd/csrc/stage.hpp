@@ -164,8 +164,14 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     fa.s = s;
     {
         TimedLaunch tl(c, HOWMHD_K_FACES);
-        const dim3 fgrid(blocks_for(c->g.n[0], FACE_TX), blocks_for(c->g.n[1], FACE_TY), c->g.n[2]);
-        hipLaunchKernelGGL((k_faces<R, EX>), fgrid, dim3(256), 0, c->stream, fa);
+        if (c->ctf.use_x && c->ctf.use_y && c->ctf.use_z) {
+            const dim3 fgrid(blocks_for(c->g.n[0], FACE_TX), blocks_for(c->g.n[1], FACE_TY),
+                             blocks_for(c->g.n[2], FACE_CZ));
+            hipLaunchKernelGGL((k_faces_march<R, EX>), fgrid, dim3(256), 0, c->stream, fa);
+        } else {
+            const dim3 fgrid(blocks_for(c->g.n[0], FACE_TX), blocks_for(c->g.n[1], FACE_TY), c->g.n[2]);
+            hipLaunchKernelGGL((k_faces<R, EX>), fgrid, dim3(256), 0, c->stream, fa);
+        }
     }
     HCHECK(c, hipGetLastError());
 
