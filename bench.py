@@ -46,7 +46,7 @@ BYTES_CELL = {
     "emf2": 8 * (6 + 3),          # 6 CT flux arrays in, 3 EMF candidates out
     "emf": 8 * (3 + 3),           # candidates in, smoothed EMFs out (k_emf)
     "emf_fused": 8 * (6 + 3),     # 3-D grids: k_emf_march, CT fluxes in, smoothed EMFs out
-    "faces": 8 * (3 + 3 + 3),     # O in, previous + initial faces in, faces out
+    "faces": 8 * (3 + 6 + 3),     # O in, previous + initial faces in, faces out
     "combine": 8 * (15 + 5 + 5 + 3 + 8),  # 3x5 fluxes, q_init and q_prev without B, faces in; q out
     "cfl": 24.0 / 256,            # tstep folds 3 maxima per 256-cell block of the stage-5 combination
 }
