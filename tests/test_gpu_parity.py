@@ -37,6 +37,9 @@ CASES = {
     # two z chunks of the 3-D EMF kernel, the second one plane deep; ragged
     # x/y tiles
     "tall33": ("mpi_smooth3d", dict(nx=10, ny=7, nz=33), 2),
+    # 3-cell-wide column: tiles of the 3-D EMF and face kernels mostly idle,
+    # every x stencil wraps
+    "thin3": ("mpi_smooth3d", dict(nx=3, ny=4, nz=40), 3),
 }
 
 
