@@ -236,7 +236,7 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
         u1[0] = luf[0][s1].u; u1[1] = lo[O_VN][s1]; u1[2] = lo[O_VT1][s1]; u1[3] = lo[O_VT2][s1];
         u1[4] = lo[O_BN][s1]; u1[5] = luf[4][s1].u; u1[6] = luf[5][s1].u; u1[7] = lo[O_P][s1]; u1[8] = lo[O_H][s1];
     }
-        Eig<R> e;
+    Eig<R> e;
     eig_scalars<R>(gam, u0, u1, e);
     // the unscaled mode normalises the L rows with div_pos: needs 1/cs2 > 0
     if constexpr (DM == DIV_UNSCALED) bad |= __ballot(!(e.nrm_half > R(0)));

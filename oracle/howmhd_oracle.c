@@ -21,6 +21,20 @@
 #include <omp.h>
 #endif
 
+/* Test hook: square-root operands of the sweep (cell and interface
+ * eigenvalues and eigenvector normalisations) outside [2^-767, inf) and not
+ * +0, where the correctly rounded square root must scale its operand or
+ * return it as it is.  It lets a test show that its data reaches such
+ * operands (the tiny-field parity case). */
+static long long g_odd_sqrt = 0;
+static double sqrt_n(double x) {
+    if (!((x >= 0x1p-767 && x < INFINITY) || (x == 0.0 && !signbit(x)))) {
+#pragma omp atomic
+        g_odd_sqrt++;
+    }
+    return sqrt(x);
+}
+
 #define NVAR 8
 #define NSTAGE 6
 
@@ -283,10 +297,10 @@ static void line_center(line_ws* w, double gam) {
         const double bbn2 = BB2 / rho;
         const double bnx2 = Bx * Bx / rho;
         const double cs2 = fmax(0.0, gam * fabs(pg / rho));
-        const double root = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
-        const double lf = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + root)));
-        const double la = sqrt(fmax(0.0, bnx2));
-        const double ls = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 - root)));
+        const double root = sqrt_n(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+        const double lf = sqrt_n(fmax(0.0, 0.5 * (bbn2 + cs2 + root)));
+        const double la = sqrt_n(fmax(0.0, bnx2));
+        const double ls = sqrt_n(fmax(0.0, 0.5 * (bbn2 + cs2 - root)));
         (void)vv2;
         WA(w, 0, ii) = vx - lf; WA(w, 1, ii) = vx - la; WA(w, 2, ii) = vx - ls; WA(w, 3, ii) = vx;
         WA(w, 4, ii) = vx + ls; WA(w, 5, ii) = vx + la; WA(w, 6, ii) = vx + lf;
@@ -323,31 +337,31 @@ static void eigensystem(double gam, const double* ul, const double* ur, double* 
     const double bbn2 = BB2 / rho;
     const double bnx2 = Bx * Bx / rho;
     const double cs2 = (gam - 1.0) * (HH - 0.5 * (vv2 + bbn2));
-    const double cs = sqrt(fmax(0.0, cs2));
-    const double root = sqrt(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
-    const double lf = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 + root)));
-    const double la = sqrt(fmax(0.0, bnx2));
-    const double ls = sqrt(fmax(0.0, 0.5 * (bbn2 + cs2 - root)));
+    const double cs = sqrt_n(fmax(0.0, cs2));
+    const double root = sqrt_n(fmax(0.0, (bbn2 + cs2) * (bbn2 + cs2) - 4.0 * bnx2 * cs2));
+    const double lf = sqrt_n(fmax(0.0, 0.5 * (bbn2 + cs2 + root)));
+    const double la = sqrt_n(fmax(0.0, bnx2));
+    const double ls = sqrt_n(fmax(0.0, 0.5 * (bbn2 + cs2 - root)));
 
     const double Bt2 = By * By + Bz * Bz;
     const double sgnBx = sign1(Bx);
     double bty, btz;
     if (Bt2 >= 1.0e-30) {
-        bty = By / sqrt(Bt2);
-        btz = Bz / sqrt(Bt2);
+        bty = By / sqrt_n(Bt2);
+        btz = Bz / sqrt_n(Bt2);
     } else {
         bty = 1.0 / sqrt(2.0);
         btz = 1.0 / sqrt(2.0);
     }
     double af, as;
     if ((lf * lf - ls * ls) >= 1.0e-30) {
-        af = sqrt(fmax(0.0, cs2 - ls * ls)) / sqrt(lf * lf - ls * ls);
-        as = sqrt(fmax(0.0, lf * lf - cs2)) / sqrt(lf * lf - ls * ls);
+        af = sqrt_n(fmax(0.0, cs2 - ls * ls)) / sqrt_n(lf * lf - ls * ls);
+        as = sqrt_n(fmax(0.0, lf * lf - cs2)) / sqrt_n(lf * lf - ls * ls);
     } else {
         af = 1.0;
         as = 1.0;
     }
-    const double sqrtrho = sqrt(rho);
+    const double sqrtrho = sqrt_n(rho);
     const double invsqrtrho = 1.0 / sqrtrho;
 
 #define LL(r, col) L[(r) * 7 + (col)]
@@ -1100,6 +1114,12 @@ double howmhd_ref_upwind5(const double* F, const double* U, double alpha) { retu
 long long howmhd_ref_large_t_count(int reset) {
     const long long n = g_large_t;
     if (reset) g_large_t = 0;
+    return n;
+}
+
+long long howmhd_ref_odd_sqrt_count(int reset) {
+    const long long n = g_odd_sqrt;
+    if (reset) g_odd_sqrt = 0;
     return n;
 }
 

@@ -75,6 +75,9 @@ int howmhd_ref_set_route(howmhd_ref_ctx* c, int route);
 /* Test hook: WENO weight evaluations so far with some t_k = (eps + IS_k)^2
  * above 1e200 or not finite (reset != 0: start again from zero). */
 long long howmhd_ref_large_t_count(int reset);
+/* Test hook: sweep square-root operands so far outside [2^-767, inf) and not
+ * +0 (reset != 0: start again from zero). */
+long long howmhd_ref_odd_sqrt_count(int reset);
 
 /* Indicator label source for one line (qline as in howmhd_ref_line_sweep,
  * not modified): maxdev[i], i = 0..nn, the largest deviation of the WENO

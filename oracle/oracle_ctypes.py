@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
             "howmhd_ref_upwind5": (C.c_double, [_DP, _DP, C.c_double]),
             "howmhd_ref_set_route": (C.c_int, [C.c_void_p, C.c_int]),
             "howmhd_ref_large_t_count": (C.c_longlong, [C.c_int]),
+            "howmhd_ref_odd_sqrt_count": (C.c_longlong, [C.c_int]),
             "howmhd_ref_line_maxdev": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, _DP, _DP]),
         }
         for fn, (res, args) in table.items():
@@ -176,3 +177,8 @@ def large_t_count(reset=False):
     """WENO weight evaluations so far with some t_k = (eps + IS_k)^2 above
     1e200 or not finite."""
     return int(lib().howmhd_ref_large_t_count(int(reset)))
+
+
+def odd_sqrt_count(reset=False):
+    """Sweep square-root operands so far outside [2^-767, inf) and not +0."""
+    return int(lib().howmhd_ref_odd_sqrt_count(int(reset)))

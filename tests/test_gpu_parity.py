@@ -116,6 +116,40 @@ def test_fp64_bitwise_large_weno_operands(problem, kw, require_gpu):
     assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
 
 
+@pytest.mark.parametrize("problem,kw", [
+    ("mhd_blast3d", dict(nx=16, ny=16, nz=16)),
+    ("orszag_tang", dict(nx=32, ny=32, nz=1, cour=0.8)),
+])
+def test_fp64_bitwise_small_sqrt_operands(problem, kw, require_gpu):
+    """The same flow with B x 1e-160: B^2 is subnormal, so square-root
+    operands of the eigensystem (c_a^2 = B_n^2/rho and the like) fall below
+    2^-767, where the correctly rounded square root scales its operand, and
+    the Alfven and slow speeds and B_t normalisations are subnormal-derived.
+    Still bit for bit."""
+    run, q0, p = _setup(problem, kw)
+    q0 = q0.copy()
+    q0[[4, 5, 6]] *= 1e-160
+    print(f"[parity fp64 small sqrt operands] {problem} {kw}", flush=True)
+    ref = O.RefSolver(p)
+    gpu = H.GpuSolver(p)
+    for s in (ref, gpu):
+        s.upload_q0(q0)
+        s.init_faces()
+    O.odd_sqrt_count(reset=True)
+    for step in range(2):
+        dt = ref.tstep()
+        assert gpu.tstep() == dt, f"dt differs at step {step}"
+        ref.ssprk(dt)
+        gpu.ssprk(dt)
+    nodd = O.odd_sqrt_count()
+    print("square-root operands outside [2^-767, inf):", nodd)
+    assert nodd > 0
+    qr, qg = ref.q0(), gpu.q0()
+    assert np.all(np.isfinite(H.active(qr)))
+    assert _bitwise_mismatch(qg, qr) == 0
+    assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp64fast", "fp32"])
 def test_dt_from_fused_cfl_maxima(precision, require_gpu):
     """After a step, tstep takes the CFL maxima that the stage-5 combination

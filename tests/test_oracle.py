@@ -296,6 +296,25 @@ def test_floors_bound_pressure():
     assert pg.min() >= 2.0 * (1 - 1e-12)
 
 
+def test_odd_sqrt_hook_counts_only_out_of_range_operands():
+    """The test hook behind the GPU parity case with a subnormal field: no
+    count on an ordinary MHD flow (B_n = 0 gives +0 operands, which are in
+    range); counts once B^2 is subnormal.  The result itself does not depend
+    on the hook."""
+    run, p, s, q0 = make_solver("mhd_blast3d", nx=8, ny=8, nz=8)
+    O.odd_sqrt_count(reset=True)
+    s.ssprk(s.tstep())
+    assert O.odd_sqrt_count() == 0
+    q1 = q0.copy()
+    q1[[4, 5, 6]] *= 1e-160
+    s2 = O.RefSolver(p)
+    s2.upload_q0(q1)
+    s2.init_faces()
+    s2.ssprk(s2.tstep())
+    assert O.odd_sqrt_count() > 0
+    assert np.all(np.isfinite(H.active(s2.q0())))
+
+
 # ------------------------------------------------------------ convergence
 def _run_to(s, tend):
     t = 0.0
