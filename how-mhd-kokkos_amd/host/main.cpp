@@ -5,8 +5,12 @@
 // library, build the face field, write output 0, then for each output
 // interval run CFL-limited hydro steps (dt clipped to the next output time,
 // abort on dt <= 0 or a non-finite time) and write the output.  Added: `nstep`
-// > 0 stops after that many hydro steps, and a closing [timing] line reports
-// zone-cycles/s of the step loop.
+// > 0 stops after that many hydro steps, a closing [timing] line reports
+// zone-cycles/s of the step loop, `checkpoint_every N` writes
+// checkpoint/ckptNNNNNN.bin (q(0), faces, output step, steps, time) after
+// every N-th output, and `restart <file>` resumes from such a file with the
+// next output interval.  A resumed run continues the uninterrupted one bit for
+// bit (the learned indicator's plain-WENO fallback starts afresh).
 #include "howmhd_gpu.h"
 #include "howmhd_host.h"
 #include "howmhd_mlp.h"
@@ -15,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <sstream>
@@ -93,8 +98,21 @@ int main(int argc, char* argv[]) {
     if (howmhd_host_gpu_params(&par, &gp, err, sizeof(err)) != 0) die(err);
 
     const size_t ncell = (size_t)(par.nx + 6) * (par.ny + 6) * (par.nz + 6);
-    std::vector<double> q0(8 * ncell, 0.0);
-    if (howmhd_host_init_problem(&par, q0.data(), err, sizeof(err)) != 0) die(err);
+    std::vector<double> q0(8 * ncell, 0.0), faces;
+    const bool resume = par.restart[0] != '\0';
+    int32_t start_out = 0;
+    int64_t steps_done = 0;
+    double time = 0.0;
+    if (resume) {
+        faces.assign(3 * ncell, 0.0);
+        if (howmhd_host_read_checkpoint(&par, par.restart, q0.data(), faces.data(), &start_out, &steps_done, &time, err,
+                                        sizeof(err)) != 0)
+            die(err);
+        std::cout << "restart            = " << par.restart << " (output step " << start_out << ", time " << time
+                  << ", " << steps_done << " hydro steps)\n";
+    } else if (howmhd_host_init_problem(&par, q0.data(), err, sizeof(err)) != 0) {
+        die(err);
+    }
     // reject an unknown output format before any step runs (src/output.cpp:565-569)
     {
         const std::string f = par.output_format;
@@ -114,9 +132,9 @@ int main(int argc, char* argv[]) {
         std::cout << "indicator weights  = " << par.indicator_weights << " (threshold " << thr << ")\n";
     }
     check(howmhd_gpu_upload_q0(ctx, q0.data()), ctx, "upload_q0");
-    check(howmhd_gpu_init_faces(ctx), ctx, "init_faces");
+    if (resume) check(howmhd_gpu_upload_faces(ctx, faces.data()), ctx, "upload_faces");
+    else check(howmhd_gpu_init_faces(ctx), ctx, "init_faces");
 
-    double time = 0.0;
     const double dtout = par.tend / static_cast<double>(par.nt);
     std::cout << "dtout = " << dtout << std::endl;
 
@@ -126,13 +144,27 @@ int main(int argc, char* argv[]) {
         if (howmhd_host_output(&par, q0.data(), step, t, "", err, sizeof(err)) != 0) die(err);
     };
 
-    write_output(0, time);
-    std::cout << "output step = 0 time = " << time << std::endl;
+    auto write_checkpoint = [&](int out_step, long steps, double t) {
+        char name[64];
+        std::snprintf(name, sizeof(name), "checkpoint/ckpt%06d.bin", out_step);
+        std::filesystem::create_directories("checkpoint");
+        faces.resize(3 * ncell);
+        check(howmhd_gpu_download_q0(ctx, q0.data()), ctx, "download_q0");
+        check(howmhd_gpu_download_faces(ctx, faces.data()), ctx, "download_faces");
+        if (howmhd_host_write_checkpoint(&par, name, q0.data(), faces.data(), out_step, steps, t, err, sizeof(err)) != 0)
+            die(err);
+        std::cout << "checkpoint = " << name << std::endl;
+    };
 
-    long total_steps = 0;
+    if (!resume) {
+        write_output(0, time);
+        std::cout << "output step = 0 time = " << time << std::endl;
+    }
+
+    long total_steps = (long)steps_done;
     double loop_seconds = 0.0;
     bool stop = false;
-    for (int out_step = 1; out_step <= par.nt && !stop; ++out_step) {
+    for (int out_step = start_out + 1; out_step <= par.nt && !stop; ++out_step) {
         const double target_time = dtout * static_cast<double>(out_step);
         int hydro_step = 0;
         double dt = 0.0;
@@ -171,10 +203,13 @@ int main(int argc, char* argv[]) {
         write_output(out_step, time);
         std::cout << "output step = " << out_step << " time = " << time << " last_dt = " << dt
                   << " hydro_steps = " << hydro_step << std::endl;
+        if (!stop && par.checkpoint_every > 0 && out_step % par.checkpoint_every == 0)
+            write_checkpoint(out_step, total_steps, time);
     }
     const double zones = (double)par.gnx * par.gny * par.gnz;
-    std::cout << "[timing] hydro_steps = " << total_steps << " loop_seconds = " << loop_seconds
-              << " zone_cycles_per_s = " << (loop_seconds > 0 ? zones * total_steps / loop_seconds : 0.0)
+    const long run_steps = total_steps - (long)steps_done;  // steps of this invocation
+    std::cout << "[timing] hydro_steps = " << run_steps << " loop_seconds = " << loop_seconds
+              << " zone_cycles_per_s = " << (loop_seconds > 0 ? zones * run_steps / loop_seconds : 0.0)
               << std::endl;
     howmhd_gpu_destroy(ctx);
     return 0;

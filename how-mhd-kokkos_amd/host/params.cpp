@@ -5,8 +5,9 @@
 // operator>>, a token starting with '#' comments out the rest of its line,
 // unknown keys warn on stderr and skip the rest of their line, and nx/ny/nz are
 // the global active-cell counts.  Keys this implementation adds: precision,
-// indicator, indicator_weights, seed, quiet; nstep gets the meaning "stop
-// after nstep hydro steps" (the reference parses it and never reads it).
+// indicator, indicator_weights, seed, quiet, checkpoint_every, restart; nstep
+// gets the meaning "stop after nstep hydro steps" (the reference parses it and
+// never reads it).
 #include "howmhd_host.h"
 
 #include <cmath>
@@ -86,6 +87,8 @@ extern "C" int howmhd_host_parse(const char* text, howmhd_run* r, char* errbuf, 
         else if (key == "indicator_weights") { in >> s; set_str(r->indicator_weights, sizeof(r->indicator_weights), s); }
         else if (key == "seed") { unsigned long long v = 0; in >> v; r->seed = v; }
         else if (key == "quiet") in >> r->quiet;
+        else if (key == "checkpoint_every") in >> r->checkpoint_every;
+        else if (key == "restart") { in >> s; set_str(r->restart, sizeof(r->restart), s); }
         else {
             std::cerr << "Warning: unknown parameter " << key << "\n";
             std::string dummy;

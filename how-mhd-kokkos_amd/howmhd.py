@@ -72,6 +72,8 @@ class Run(C.Structure):
         ("indicator_weights", C.c_char * 256),
         ("seed", C.c_uint64),
         ("quiet", C.c_int32),
+        ("checkpoint_every", C.c_int32),
+        ("restart", C.c_char * 256),
     ]
 
 
@@ -137,6 +139,10 @@ HOST_FUNCS = {
     "howmhd_host_output": (C.c_int, [C.POINTER(Run), _DP, C.c_int, C.c_double, C.c_char_p, C.c_char_p, C.c_int]),
     "howmhd_host_load_indicator": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
                                              C.c_char_p, C.c_int]),
+    "howmhd_host_write_checkpoint": (C.c_int, [C.POINTER(Run), C.c_char_p, _DP, _DP, C.c_int32, C.c_int64,
+                                               C.c_double, C.c_char_p, C.c_int]),
+    "howmhd_host_read_checkpoint": (C.c_int, [C.POINTER(Run), C.c_char_p, _DP, _DP, C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_char_p, C.c_int]),
 }
 
 
@@ -212,6 +218,30 @@ def load_indicator(path) -> tuple[np.ndarray, float]:
                                       C.byref(thr), err, len(err)) != 0:
         raise HowmhdError(err.value.decode())
     return P, thr.value
+
+
+def write_checkpoint(run: Run, path, q0: np.ndarray, b: np.ndarray, out_step: int, total_steps: int,
+                     time: float) -> None:
+    """howmhd_host_write_checkpoint: q(0), faces and the loop position."""
+    err = C.create_string_buffer(4096)
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    if host_lib().howmhd_host_write_checkpoint(C.byref(run), str(path).encode(), _ptr(q0), _ptr(b), out_step,
+                                               total_steps, time, err, len(err)) != 0:
+        raise HowmhdError(err.value.decode())
+
+
+def read_checkpoint(run: Run, path):
+    """howmhd_host_read_checkpoint -> (q0, faces, out_step, total_steps, time)."""
+    err = C.create_string_buffer(4096)
+    shape = (run.nx + 6, run.ny + 6, run.nz + 6)
+    q0 = np.zeros((8,) + shape)
+    b = np.zeros((3,) + shape)
+    o, n, t = C.c_int32(), C.c_int64(), C.c_double()
+    if host_lib().howmhd_host_read_checkpoint(C.byref(run), str(path).encode(), _ptr(q0), _ptr(b), C.byref(o),
+                                              C.byref(n), C.byref(t), err, len(err)) != 0:
+        raise HowmhdError(err.value.decode())
+    return q0, b, o.value, n.value, t.value
 
 
 def make_input(**kw) -> str:

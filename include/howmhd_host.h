@@ -34,6 +34,8 @@ typedef struct howmhd_run {
     char indicator_weights[256];    /* weights file for indicator mlp          */
     uint64_t seed;                  /* turbulence3d random phases               */
     int32_t quiet;                  /* 1: no per-step log lines                */
+    int32_t checkpoint_every;       /* > 0: checkpoint after every such output */
+    char restart[256];              /* checkpoint file to resume from          */
 } howmhd_run;
 
 /* read_parameters + finalize_parameters_after_read
@@ -64,6 +66,23 @@ HOWMHD_API int howmhd_host_output(const howmhd_run* run, const double* q0, int s
  * must equal HOWMHD_MLP_NPARAM (1697) and the threshold lie in (0, 1). */
 HOWMHD_API int howmhd_host_load_indicator(const char* path, float* params, int nparam, float* threshold,
                                char* errbuf, int errlen);
+
+/* Checkpoint of a run: q(0) and the stage-0 faces in the reference layout
+ * (8 and 3 arrays of (nx+6) x (ny+6) x (nz+6) doubles, k fastest), the output
+ * step reached, the hydro steps taken and the time.  No counterpart in the
+ * reference, which has no checkpoint/restart; for runs longer than one job
+ * (the `checkpoint_every` and `restart` input keys).  Binary, native byte
+ * order, a 48-byte header ("HOWMHDCK", u32 version 1, i32 nx, ny, nz,
+ * i32 out_step, 4 bytes of padding, i64 total_steps, f64 time) and then q0 and
+ * the faces.  Writing creates or
+ * replaces `path` (its directory must exist); reading checks the magic, the
+ * version and that nx, ny, nz are run's. */
+HOWMHD_API int howmhd_host_write_checkpoint(const howmhd_run* run, const char* path, const double* q0,
+                                 const double* b, int32_t out_step, int64_t total_steps, double time,
+                                 char* errbuf, int errlen);
+HOWMHD_API int howmhd_host_read_checkpoint(const howmhd_run* run, const char* path, double* q0, double* b,
+                                int32_t* out_step, int64_t* total_steps, double* time, char* errbuf,
+                                int errlen);
 
 #ifdef __cplusplus
 }

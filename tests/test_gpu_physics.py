@@ -238,3 +238,40 @@ quiet 1
     l1 = np.abs(rho - want).sum() / np.abs(want).sum()
     print("driver fp32+indicator vs FP64 restatement, density rel L1:", l1)
     assert l1 <= 1e-3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_driver_restart_continues_bit_for_bit(tmp_path, precision, require_gpu):
+    """how_mhd with `checkpoint_every 2` over 4 outputs, and a second run that
+    resumes from the output-2 checkpoint: outputs 3 and 4 of the two runs are
+    the same bytes (q(0) and faces travel through the checkpoint exactly; the
+    first tstep after the restart takes its CFL maxima from q(0) itself)."""
+    text = f"""problem mhd_blast3d
+output_format vts
+nx 16
+ny 12
+nz 10
+gam 1.6666666666666667
+cour 0.4
+tend 0.02
+nt 4
+precision {precision}
+quiet 1
+"""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    (a / "input.in").write_text(text + "checkpoint_every 2\n")
+    ra = subprocess.run([str(H.DRIVER), "input.in"], cwd=a, capture_output=True, text=True, timeout=120)
+    assert ra.returncode == 0, ra.stderr
+    ckpt = a / "checkpoint" / "ckpt000002.bin"
+    assert ckpt.exists() and (a / "checkpoint" / "ckpt000004.bin").exists()
+    (b / "input.in").write_text(text + f"restart {ckpt}\n")
+    rb = subprocess.run([str(H.DRIVER), "input.in"], cwd=b, capture_output=True, text=True, timeout=120)
+    assert rb.returncode == 0, rb.stderr
+    assert "restart" in rb.stdout and "output step = 3" in rb.stdout
+    assert sorted(f.name for f in (b / "output").glob("dump*.vts")) == [
+        "dump000003_rank000000.vts", "dump000004_rank000000.vts"]
+    for k in (3, 4):
+        name = f"dump{k:06d}_rank000000.vts"
+        assert (a / "output" / name).read_bytes() == (b / "output" / name).read_bytes(), name

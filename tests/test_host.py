@@ -271,3 +271,35 @@ def test_indicator_weights_roundtrip(tmp_path):
     bad.write_text("howmhd-mlp 1697 1.5\n")
     with pytest.raises(H.HowmhdError, match="threshold"):
         H.load_indicator(bad)
+
+
+def test_checkpoint_roundtrip_and_checks(tmp_path):
+    """Checkpoint files (no reference counterpart): the state and loop position
+    come back bit for bit; a file of another grid, a file that is not a
+    checkpoint and a truncated one are refused with a message."""
+    run = H.parse(H.make_input(problem="orszag_tang", nx=10, ny=7, nz=1, nt=2, tend=1.0, checkpoint_every=1,
+                               restart="prev.bin"))
+    assert run.checkpoint_every == 1 and run.restart == b"prev.bin"
+    rng = np.random.default_rng(7)
+    shape = (run.nx + 6, run.ny + 6, run.nz + 6)
+    q, b = rng.standard_normal((8,) + shape), rng.standard_normal((3,) + shape)
+    q[0, 3, 3, 3] = -0.0
+    path = tmp_path / "c.bin"
+    H.write_checkpoint(run, path, q, b, 3, 1234567890123, 0.1 + 0.2)
+    assert path.stat().st_size == 48 + 8 * 11 * q[0].size
+    q2, b2, o, n, t = H.read_checkpoint(run, path)
+    assert np.array_equal(q.view(np.int64), q2.view(np.int64)) and np.array_equal(b, b2)
+    assert (o, n, t) == (3, 1234567890123, 0.1 + 0.2)
+    other = H.parse(H.make_input(problem="orszag_tang", nx=10, ny=8, nz=1, nt=2, tend=1.0))
+    with pytest.raises(H.HowmhdError, match="does not match"):
+        H.read_checkpoint(other, path)
+    junk = tmp_path / "junk.bin"
+    junk.write_bytes(b"not a checkpoint at all" * 4)
+    with pytest.raises(H.HowmhdError, match="not a checkpoint"):
+        H.read_checkpoint(run, junk)
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(path.read_bytes()[:1000])
+    with pytest.raises(H.HowmhdError, match="truncated"):
+        H.read_checkpoint(run, cut)
+    with pytest.raises(H.HowmhdError, match="cannot open"):
+        H.read_checkpoint(run, tmp_path / "missing.bin")
