@@ -426,6 +426,8 @@ int howmhd_gpu_diag(howmhd_gpu_ctx* c, howmhd_diag* d) {
                 const double p = (gam - 1.0) * (q[7 * nc + i] - 0.5 * rho * (vx * vx + vy * vy + vz * vz) - 0.5 * BB2);
                 d->min_rho = std::min(d->min_rho, rho);
                 d->min_p = std::min(d->min_p, p);
+                if (rho == c->p.rhomin) d->n_rho_floor++;
+                if (std::fabs(p - c->p.pgmin) <= 1e-12 * std::fabs(c->p.pgmin)) d->n_p_floor++;
                 // CT face divergence: D4 of src/fluxct.cpp:848 along each axis
                 double dv = 0.0;
                 const int P0 = c->g.per[0], P1 = c->g.per[1], P2 = c->g.per[2];

@@ -44,6 +44,7 @@ class Diag(C.Structure):
         ("max_divb_cell", C.c_double),
         ("min_rho", C.c_double), ("min_p", C.c_double),
         ("flagged_fraction", C.c_double),
+        ("n_rho_floor", C.c_int64), ("n_p_floor", C.c_int64),
     ]
 
 

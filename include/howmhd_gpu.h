@@ -76,6 +76,9 @@ typedef struct howmhd_diag {
     double max_divb_cell;    /* max |2nd-order cell-centred divB| (output.cpp:314-347) */
     double min_rho, min_p;
     double flagged_fraction; /* indicator: flagged interfaces / all, last step   */
+    /* cells at the floors of prot (src/prot.cpp:31, 41): rho == rhomin, and
+     * p = (gam-1)(E - kinetic - magnetic) within 1e-12 (relative) of pgmin */
+    int64_t n_rho_floor, n_p_floor;
 } howmhd_diag;
 
 /* Per-kernel device time of the launches issued while timing was on,

@@ -95,6 +95,28 @@ def test_conservation_and_ct_divergence_product_build(require_gpu):
     assert ddiv <= 1e-13 * scale
 
 
+def test_diag_floor_counts(require_gpu):
+    """howmhd_gpu_diag counts the cells that prot left at the density and
+    pressure floors; the counts equal those of the downloaded state."""
+    # pgmin at the ambient pressure: the rarefied cells sit on the floor
+    run, q0, p = _setup("blast3d", nx=16, ny=16, nz=8, rhomin=0.9999, pgmin=1.0)
+    g = H.GpuSolver(p)
+    g.upload_q0(q0)
+    g.init_faces()
+    for _ in range(2):
+        g.ssprk(g.tstep())
+    d = g.diag()
+    q = H.active(g.q0())
+    rho = q[0]
+    vx, vy, vz = q[1] / rho, q[2] / rho, q[3] / rho
+    pg = (p.gam - 1.0) * (q[7] - 0.5 * (rho * (vx * vx + vy * vy + vz * vz) + (q[4] ** 2 + q[5] ** 2 + q[6] ** 2)))
+    n_rho = int(np.count_nonzero(rho == p.rhomin))
+    n_p = int(np.count_nonzero(np.abs(pg - p.pgmin) <= 1e-12 * p.pgmin))
+    print("floor counts:", d.n_rho_floor, d.n_p_floor, "expected", n_rho, n_p)
+    assert n_rho > 0 and n_p > 0
+    assert (d.n_rho_floor, d.n_p_floor) == (n_rho, n_p)
+
+
 def _alfven_error(n, solver_cls, **kw):
     run, q0, p = _setup("cpaw3d", nx=n, ny=n, nz=n, xsize=3.0, ysize=1.5, zsize=1.5, cour=0.4)
     s = solver_cls(p, **kw)
