@@ -198,8 +198,8 @@ def form_leg(H, torch, run, args, fp64_value, precision, indicator=False):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              f"(single-GPU time step; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(single-GPU time step; sharding by 3-D blocks with halo exchange is not built)", file=sys.stderr)
         sys.exit(2)
     import torch  # device plumbing only: selects the GPU and synchronises
 

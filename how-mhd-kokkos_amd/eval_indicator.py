@@ -2,7 +2,7 @@
 
     python how-mhd-kokkos_amd/eval_indicator.py --out report.json [--cases rotor512,turb256,blast256]
 
-For each case the plain FP64 method (reference rounding) is the judge: its dt
+For each case the plain FP64 method (reference rounding) is the reference run: its dt
 sequence drives every form from the same initial state, and every form's
 density and energy are compared with it at the end (relative L1 over active
 cells).  Forms:
@@ -72,18 +72,18 @@ def untrained_params(seed=123):
     return P
 
 
-def matched_threshold(judge, P_trained, thr_trained, P_other):
-    """Threshold of P_other that flags the same fraction of the judge's
+def matched_threshold(ref_run, P_trained, thr_trained, P_other):
+    """Threshold of P_other that flags the same fraction of the reference run's
     current interfaces as the trained network does."""
     import torch
     feats = []
     for d in range(3):
-        n = judge.n_samples(d)
+        n = ref_run.n_samples(d)
         if n == 0:
             continue
         X = torch.empty((n, H.MLP_NFEAT), dtype=torch.float32, device="cuda")
         y = torch.empty(n, dtype=torch.float32, device="cuda")
-        judge.collect_samples(d, X, y)
+        ref_run.collect_samples(d, X, y)
         feats.append(X)
     X = torch.cat(feats)
     zt = H.mlp_infer(X, torch.from_numpy(P_trained).cuda())
@@ -106,10 +106,10 @@ def run_case(name, time_steps=10, forms=None):
     zones = run.gnx * run.gny * run.gnz
     P, thr = H.load_indicator(WEIGHTS)
 
-    # judge trajectory: plain FP64 dt sequence
-    judge = _solver(run, "fp64")
-    judge.upload_q0(q0)
-    judge.init_faces()
+    # reference trajectory: plain FP64 dt sequence
+    ref_run = _solver(run, "fp64")
+    ref_run.upload_q0(q0)
+    ref_run.init_faces()
     dts, t = [], 0.0
     tend = stop.get("tend")
     nsteps = stop.get("steps")
@@ -120,21 +120,21 @@ def run_case(name, time_steps=10, forms=None):
             break
         if tend is not None and t >= tend * (1 - 1e-12):
             break
-        dt = judge.tstep()
+        dt = ref_run.tstep()
         if tend is not None:
             dt = min(dt, tend - t)
-        judge.ssprk(dt)
+        ref_run.ssprk(dt)
         dts.append(dt)
         t += dt
         half = (nsteps is not None and len(dts) == nsteps // 2) or (tend is not None and t >= 0.5 * tend)
         if thr_un is None and half:
             # the untrained network's threshold, matched on the mid-run state
-            thr_un, frac_cal = matched_threshold(judge, P, thr, P_un)
+            thr_un, frac_cal = matched_threshold(ref_run, P, thr, P_un)
     if thr_un is None:
-        thr_un, frac_cal = matched_threshold(judge, P, thr, P_un)
-    q_judge = judge.q0()
-    judge.close()
-    log(f"[{name}] judge: {len(dts)} steps to t = {t:.5g}")
+        thr_un, frac_cal = matched_threshold(ref_run, P, thr, P_un)
+    q_ref = ref_run.q0()
+    ref_run.close()
+    log(f"[{name}] reference: {len(dts)} steps to t = {t:.5g}")
 
     forms = forms or ["fp64", "fp32", "fp32+indicator", "fp32+linear", "fp32+untrained", "fp64+indicator"]
     P_lin = P.copy()       # output weights 0, bias -50: nothing is flagged
@@ -171,7 +171,7 @@ def run_case(name, time_steps=10, forms=None):
         speed = zones * time_steps / el
         if form == "fp64":
             base_speed = speed
-        # accuracy along the judge's dt sequence (same routing mode)
+        # accuracy along the reference run's dt sequence (same routing mode)
         if Pf is not None:
             g.set_indicator(Pf, thf, enable=mode)
         g.upload_q0(q0)
@@ -182,7 +182,7 @@ def run_case(name, time_steps=10, forms=None):
         qf = g.q0()
         if not np.all(np.isfinite(H.active(qf))):
             ok = False
-        l1 = H.rel_l1(qf, q_judge)
+        l1 = H.rel_l1(qf, q_ref)
         ent = {"zone_cycles_per_s": round(speed, 1), "ms_per_step": round(el / time_steps * 1e3, 3),
                "rel_l1_rho": float(l1[0]), "rel_l1_E": float(l1[7]), "finite": ok,
                "divb_ct_max": g.diag().max_divb_ct}

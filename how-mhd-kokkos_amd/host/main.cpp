@@ -88,7 +88,7 @@ int main(int argc, char* argv[]) {
     howmhd_run par;
     if (howmhd_host_parse(text.c_str(), &par, err, sizeof(err)) != 0) die(err);
     if (par.px > 1 || par.py > 1 || par.pz > 1)
-        std::cerr << "Warning: px/py/pz were provided, but this build runs one rank on one GPU. "
+        std::cerr << "Warning: px/py/pz were provided, but this program runs one rank on one GPU. "
                   << "Ignoring MPI layout and using one rank.\n";
 
     print_banner();

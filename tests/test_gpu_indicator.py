@@ -233,8 +233,8 @@ def test_indicator_mode_requires_weights(require_gpu):
 
 def test_trained_indicator_ablation(require_gpu):
     """What the trained network gains over the method without it, on the MHD
-    rotor (128^2, gam 1.4, to t = 0.05; the judge is the plain FP64 method):
-      * FP32 + trained indicator: density within 1e-3 of the judge, with a
+    rotor (128^2, gam 1.4, to t = 0.05; the reference is the plain FP64 method):
+      * FP32 + trained indicator: density within 1e-3 of the reference, with a
         minority of interfaces on full WENO;
       * every interface on linear upwind-5 (no indicator): >= 10x its error;
       * an untrained network flagging the same fraction: >= 10x its error."""

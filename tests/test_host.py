@@ -303,3 +303,14 @@ def test_checkpoint_roundtrip_and_checks(tmp_path):
         H.read_checkpoint(run, cut)
     with pytest.raises(H.HowmhdError, match="cannot open"):
         H.read_checkpoint(run, tmp_path / "missing.bin")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message before
+    touching a device."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 2
+    assert "measures one GPU" in out.stderr and out.stdout == ""
