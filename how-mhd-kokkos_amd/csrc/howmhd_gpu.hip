@@ -221,8 +221,11 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
         }
     }
     for (int t = 0; t < 6 && ok; ++t) ok = ok && alloc(&c->ct[t], c->ncell * c->rsize);
+    // the unsmoothed EMFs O2 live in memory only on 1-D/2-D grids; on 3-D
+    // grids k_emf_march keeps them on chip
+    const bool emf_fused = c->ctf.use_x && c->ctf.use_y && c->ctf.use_z;
     for (int t = 0; t < 3 && ok; ++t) {
-        ok = ok && alloc(&c->O2[t], c->ncell * c->rsize);
+        if (!emf_fused) ok = ok && alloc(&c->O2[t], c->ncell * c->rsize);
         ok = ok && alloc(&c->O[t], c->ncell * c->rsize);
     }
     ok = ok && alloc((void**)&c->dmax, 4 * sizeof(unsigned long long));
