@@ -75,10 +75,14 @@ struct howmhd_gpu_ctx {
     std::string err = "no error";
 };
 
+// A failed HIP call is reported once: the runtime's last-error state is
+// cleared, so that the next launch check of this thread does not report it
+// again.
 #define HCHECK(ctx, call)                                                                  \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \
         if (e_ != hipSuccess) {                                                            \
+            (void)hipGetLastError();                                                       \
             (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
             return HOWMHD_EHIP;                                                            \
         }                                                                                  \

@@ -52,7 +52,10 @@ int fold_indicator_stats(howmhd_gpu_ctx* c, bool wait) {
         HCHECK(c, hipEventSynchronize(c->istats_ev));
     } else {
         const hipError_t q = hipEventQuery(c->istats_ev);
-        if (q == hipErrorNotReady) return HOWMHD_OK;
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError();  // not an error: the copy has not arrived yet
+            return HOWMHD_OK;
+        }
         HCHECK(c, q);
     }
     long long s[4];
@@ -198,6 +201,7 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     c->rsize = (p->precision == HOWMHD_PREC_FP32) ? sizeof(float) : sizeof(double);
 
     auto cleanup_fail = [&](int code, const std::string& msg) {
+        (void)hipGetLastError();  // reported here, not at a later launch check
         g_create_error = msg;
         howmhd_gpu_destroy(c);
         return code;
@@ -205,9 +209,14 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     if (hipGetDevice(&c->device) != hipSuccess) return cleanup_fail(HOWMHD_EHIP, "hipGetDevice failed (no GPU?)");
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(HOWMHD_EHIP, "hipStreamCreate failed");
+    // a failed allocation is handled here: the runtime's last error is
+    // cleared so that it does not surface at a later launch check
     auto alloc = [&](void** ptr, size_t bytes) -> bool {
-        if (hipMalloc(ptr, bytes) != hipSuccess) return false;
-        return hipMemset(*ptr, 0, bytes) == hipSuccess;
+        if (hipMalloc(ptr, bytes) != hipSuccess || hipMemset(*ptr, 0, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        return true;
     };
     bool ok = true;
     for (int b = 0; b < 5 && ok; ++b) {
@@ -342,6 +351,10 @@ int howmhd_gpu_tstep(howmhd_gpu_ctx* c, double* dt) {
 
 int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
     if (!c) return HOWMHD_EINVAL;
+    // the reference's loop stops before stepping with dt <= 0
+    // (src/main.cpp:175-182); a negative or non-finite dt is refused here
+    if (!(dt >= 0.0) || !std::isfinite(dt))
+        return fail(c, HOWMHD_EINVAL, "howmhd_gpu_ssprk: dt must be finite and >= 0");
     if (!c->faces_ready) return fail(c, HOWMHD_ESTATE, "howmhd_gpu_ssprk: call init_faces or upload_faces first");
     if (c->p.indicator == HOWMHD_INDICATOR_MLP && !c->ind_mode)
         return fail(c, HOWMHD_ESTATE, "howmhd_gpu_ssprk: indicator mlp needs weights (howmhd_gpu_set_indicator)");

@@ -124,7 +124,9 @@ HOWMHD_API int howmhd_gpu_upload_faces(howmhd_gpu_ctx* ctx, const double* b);
  * q(0) (src/tstep.cpp:11-136).  Synchronises. */
 HOWMHD_API int howmhd_gpu_tstep(howmhd_gpu_ctx* ctx, double* dt);
 
-/* One SSPRK(5,4) step of size dt (src/ssprk.cpp:1277-1329). */
+/* One SSPRK(5,4) step of size dt (src/ssprk.cpp:1277-1329).  HOWMHD_EINVAL
+ * for a negative or non-finite dt; HOWMHD_ESTATE before init_faces or
+ * upload_faces, or with indicator mlp before set_indicator. */
 HOWMHD_API int howmhd_gpu_ssprk(howmhd_gpu_ctx* ctx, double dt);
 
 /* q(0) to the host in the reference layout, ghosts filled with the
