@@ -14,7 +14,7 @@ value = 256^3 * steps / wall time of the timed steps.  After the timed loop a
 short run with per-kernel HIP-event timing on the library's stream gives the
 roofline entry of the dominant kernel.  The CPU baseline is the C restatement
 of the reference time step (oracle/) on the host cores, on a bounded sample
-(the same problem at 128^3, one step).
+(the benchmark workload, one step at 256^3).
 
 Prints ONE JSON line.
 """
@@ -64,7 +64,7 @@ def parse_args():
     ap.add_argument("--precision", choices=["fp64", "fp32", "fp64fast"], default="fp64")
     ap.add_argument("--indicator", action="store_true",
                     help="time the form with the learned indicator (weights/indicator_mlp.txt) instead")
-    ap.add_argument("--cpu-size", type=int, default=128)
+    ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
