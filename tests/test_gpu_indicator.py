@@ -317,3 +317,43 @@ def test_in_sweep_network_matches_standalone_kernel(require_gpu):
         print(f"dir {d}: max |dz| {dz.max():.3e}, mean {dz.mean():.3e}")
         assert np.all(dz <= 2e-2 * scale)
         assert np.mean(dz <= 1e-4 * scale) >= 0.99
+
+
+def test_fp32_indicator_full_size_invariants(require_gpu):
+    """The product form at the benchmark size: MHD blast 256^3, FP32 with the
+    trained indicator (plain-WENO fallback on), 3 steps.  Size-independent
+    properties: mass and momentum are conserved on the periodic grid (one flux
+    per interface, FP64 stage sums), energy up to prot's rebuild from the
+    face-interpolated B, max |div_CT| stays below 1e-10
+    (FP64 face update of FP32 EMFs), the state stays finite, and the routing
+    is active (a small flagged fraction)."""
+    import eval_indicator as E
+    run, q0, p = _setup("mhd_blast3d", nx=256, ny=256, nz=256, cour=0.4)
+    p.precision = H.PREC["fp32"]
+    P, thr = H.load_indicator(E.WEIGHTS)
+    g = H.GpuSolver(p)
+    try:
+        g.set_indicator(P, thr, enable=1)
+        g.upload_q0(q0)
+        g.init_faces()
+        d0 = g.diag()
+        for _ in range(3):
+            g.ssprk(g.tstep())
+        d1 = g.diag()
+        st = g.indicator_stats()
+        print("256^3 fp32+indicator:", st, "divB_CT", d1.max_divb_ct, "min rho", d1.min_rho)
+        a0 = np.abs(H.active(q0)).reshape(8, -1).sum(axis=1)
+        # the blast starts at rest: momentum is measured against what it grew to
+        a1 = np.abs(H.active(g.q0())).reshape(8, -1).sum(axis=1)
+        for m in (0, 1, 2, 3):
+            scale = a1[1:4].sum() if m in (1, 2, 3) else a0[m]
+            assert abs(d1.sum_cons[m] - d0.sum_cons[m]) <= 1e-12 * scale, m
+        # energy: prot rebuilds E from p with the face-interpolated B
+        # (src/prot.cpp:45-46, src/fluxct.cpp:891-896), which moves E by the
+        # difference of the two magnetic energies; measured 2.5e-12 here
+        assert abs(d1.sum_cons[7] - d0.sum_cons[7]) <= 1e-10 * a0[7]
+        assert d1.max_divb_ct <= 1e-10
+        assert np.isfinite(d1.min_rho) and np.isfinite(d1.min_p) and d1.min_rho > 0 and d1.min_p > 0
+        assert st["routed_steps"] == 3 and 0.0 < st["flagged_fraction"] < 0.05
+    finally:
+        g.close()
