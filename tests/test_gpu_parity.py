@@ -40,6 +40,10 @@ CASES = {
     # 3-cell-wide column: tiles of the 3-D EMF and face kernels mostly idle,
     # every x stencil wraps
     "thin3": ("mpi_smooth3d", dict(nx=3, ny=4, nz=40), 3),
+    # three z planes: the z windows of the 3-D EMF and face kernels wrap
+    # (periodic) or clamp (open) at both ends at once
+    "flat3_per": ("mpi_smooth3d", dict(nx=12, ny=10, nz=3), 2),
+    "flat3_open": ("mhd_blast3d", dict(nx=10, ny=9, nz=3, x3bc="open"), 2),
 }
 
 
