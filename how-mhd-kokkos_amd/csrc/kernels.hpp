@@ -975,7 +975,10 @@ __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
 // next plane are loaded one plane ahead, and each column keeps Oy and Oz of
 // planes z - 3 .. z + 2 (images through bmap) in registers for the z
 // differences.  The arithmetic is k_faces', term for term.
-constexpr int FACE_CZ = 32;
+#ifndef HOWMHD_FACE_CZ
+#define HOWMHD_FACE_CZ 32
+#endif
+constexpr int FACE_CZ = HOWMHD_FACE_CZ;
 // 3 wavefronts per SIMD (136 VGPRs, no scratch in FP64; 4 spills 4 values and
 // runs 4% slower)
 #ifndef HOWMHD_FACE_WAVES
