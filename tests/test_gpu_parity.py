@@ -154,6 +154,37 @@ def test_fp64_bitwise_small_sqrt_operands(problem, kw, require_gpu):
     assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
 
 
+def test_fp64_matches_regression_fixtures(require_gpu):
+    """The plain FP64 GPU path against the committed regression fixtures of the
+    restatement (tests/golden/oracle_regression.json): the same dt at every
+    step and the same SHA-256 of q(0) and the faces, without running the
+    restatement."""
+    import hashlib
+    import json
+    from pathlib import Path
+    gdir = Path(__file__).resolve().parent / "golden"
+    want = json.loads((gdir / "oracle_regression.json").read_text())
+
+    def digest(a):
+        return hashlib.sha256(np.ascontiguousarray(H.active(a), dtype="<f8").tobytes()).hexdigest()
+
+    for case in want["cases"]:
+        print(f"[fixture] {case['problem']} {case['input']}", flush=True)
+        run = H.parse(H.make_input(problem=case["problem"], **case["input"]))
+        g = H.GpuSolver(H.gpu_params(run))
+        try:
+            g.upload_q0(H.init_problem(run))
+            g.init_faces()
+            for k in range(case["steps"]):
+                dt = g.tstep()
+                assert float(dt).hex() == case["dt"][k], (case["problem"], k)
+                g.ssprk(dt)
+            assert digest(g.q0()) == case["q_sha256"], case["problem"]
+            assert digest(g.faces()) == case["faces_sha256"], case["problem"]
+        finally:
+            g.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp64fast", "fp32"])
 def test_dt_from_fused_cfl_maxima(precision, require_gpu):
     """After a step, tstep takes the CFL maxima that the stage-5 combination

@@ -371,3 +371,23 @@ def test_convergence_order_alfven_wave_is_ct_limited():
     order = math.log2(errs[0] / errs[1])
     print("Alfven wave L1(By):", errs, "order", order)
     assert 1.6 < order < 2.6, (errs, order)
+
+
+def test_restatement_regression_fixtures():
+    """The restatement reproduces its committed regression fixtures
+    (tests/golden/oracle_regression.json: dt of every step and SHA-256 of q(0)
+    and the faces) with the default thread count, so its arithmetic is
+    unchanged and independent of the OpenMP thread count."""
+    import importlib.util
+    import json
+    from pathlib import Path
+    gdir = Path(__file__).resolve().parent / "golden"
+    spec = importlib.util.spec_from_file_location("make_oracle_fixtures", gdir / "make_oracle_fixtures.py")
+    G = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(G)
+    want = json.loads((gdir / "oracle_regression.json").read_text())
+    for case in want["cases"]:
+        got = G.run_case(case["problem"], case["input"], case["steps"])
+        assert got["dt"] == case["dt"], case["problem"]
+        assert got["q_sha256"] == case["q_sha256"], case["problem"]
+        assert got["faces_sha256"] == case["faces_sha256"], case["problem"]
