@@ -1,8 +1,8 @@
 /*
  * howmhd_oracle.c — CPU restatement of the HOW-MHD time step.
  *
- * TEST INFRASTRUCTURE ONLY: the parity checker and the CPU baseline.  Nothing
- * in the product links or loads this file.
+ * TEST INFRASTRUCTURE ONLY: the parity reference of the tests and the CPU
+ * baseline.  Nothing in the product links or loads this file.
  *
  * Every function cites the reference lines it follows.  Arithmetic is written
  * in the reference's evaluation order and the library is built with

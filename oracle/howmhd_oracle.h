@@ -2,9 +2,10 @@
  * howmhd_oracle.h — CPU restatement of the HOW-MHD time step (TEST
  * INFRASTRUCTURE ONLY).
  *
- * This library is the parity checker and the timed CPU baseline.  Only the
- * test suite, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it;
- * the product (the HIP library and the host driver) never does.
+ * This library is the parity reference of the tests and the timed CPU
+ * baseline.  Only the test suite, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg load it; the product (the HIP library and the host driver)
+ * never does.
  *
  * It restates, in plain C with the reference's operation order, the serial
  * (one-rank) paths of:

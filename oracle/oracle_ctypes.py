@@ -1,7 +1,7 @@
 """ctypes binding of the CPU restatement (TEST INFRASTRUCTURE ONLY).
 
 Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
-the parity checker; never by the product path.
+the parity reference; never by the product path.
 """
 from __future__ import annotations
 

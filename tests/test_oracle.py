@@ -1,4 +1,4 @@
-"""Pins the CPU restatement (the parity checker) against what the mathematics
+"""Pins the CPU restatement (the parity reference) against what the mathematics
 fixes.  The reference cannot be built here (it needs Kokkos) and ships no
 tests or golden data, so each piece of the restatement is checked by an
 independent property chosen so that a plausible slip (a dropped term, a wrong
