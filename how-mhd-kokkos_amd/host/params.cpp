@@ -199,8 +199,10 @@ extern "C" int howmhd_host_load_indicator(const char* path, float* params, int n
             header = true;
             continue;
         }
+        // one value per line: anything after it is an error too
         double v = 0.0;
-        if (!(in >> v) || !std::isfinite(v) || count >= nparam) {
+        std::string rest;
+        if (!(in >> v) || !std::isfinite(v) || (in >> rest) || count >= nparam) {
             fail(errbuf, errlen, std::string("Error: ") + path + ": bad or extra value: " + line);
             return HOWMHD_EINVAL;
         }

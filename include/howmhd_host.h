@@ -62,8 +62,9 @@ HOWMHD_API int howmhd_host_output(const howmhd_run* run, const double* q0, int s
 
 /* Reads a learned-indicator weights file (the `indicator_weights` input key;
  * no counterpart in the reference): text, '#' lines are comments, then the
- * header "howmhd-mlp <nparam> <threshold>" and nparam FP32 values.  nparam
- * must equal HOWMHD_MLP_NPARAM (1697) and the threshold lie in (0, 1). */
+ * header "howmhd-mlp <nparam> <threshold>" and nparam FP32 values, one per
+ * line.  nparam must equal HOWMHD_MLP_NPARAM (1697) and the threshold lie in
+ * (0, 1). */
 HOWMHD_API int howmhd_host_load_indicator(const char* path, float* params, int nparam, float* threshold,
                                char* errbuf, int errlen);
 
@@ -74,9 +75,8 @@ HOWMHD_API int howmhd_host_load_indicator(const char* path, float* params, int n
  * (the `checkpoint_every` and `restart` input keys).  Binary, native byte
  * order, a 48-byte header ("HOWMHDCK", u32 version 1, i32 nx, ny, nz,
  * i32 out_step, 4 bytes of padding, i64 total_steps, f64 time) and then q0 and
- * the faces.  Writing creates or
- * replaces `path` (its directory must exist); reading checks the magic, the
- * version and that nx, ny, nz are run's. */
+ * the faces.  Writing creates or replaces `path` (its directory must exist);
+ * reading checks the magic, the version and that nx, ny, nz are run's. */
 HOWMHD_API int howmhd_host_write_checkpoint(const howmhd_run* run, const char* path, const double* q0,
                                  const double* b, int32_t out_step, int64_t total_steps, double time,
                                  char* errbuf, int errlen);

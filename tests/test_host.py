@@ -265,6 +265,9 @@ def test_indicator_weights_roundtrip(tmp_path):
     bad.write_text("howmhd-mlp 1697 0.5\n1.0\n")
     with pytest.raises(H.HowmhdError, match="1 of 1697"):
         H.load_indicator(bad)
+    bad.write_text("howmhd-mlp 1697 0.5\n" + "1.0 2.0\n" * 1697)
+    with pytest.raises(H.HowmhdError, match="bad or extra value"):
+        H.load_indicator(bad)
     bad.write_text("howmhd-mlp 12 0.5\n")
     with pytest.raises(H.HowmhdError, match="expected 1697"):
         H.load_indicator(bad)
