@@ -38,10 +38,6 @@ struct howmhd_gpu_ctx {
     // k_combine); cfl_fresh: they belong to the current q(0)
     double* cflpart = nullptr;
     bool cfl_fresh = false;
-    // FP32 form: primitives (rho, vx, vy, vz, Bx, By, Bz, p; 8 x ncell FP32)
-    // of the stage the next sweeps read, written by each stage combination
-    // and by k_prim after an upload of q(0); null in the FP64 forms
-    float* prim = nullptr;
     // learned indicator: device weights, routing mode (0 off, 1 on with the
     // plain-WENO fallback, 2 on for every step), the routed sweeps' counters
     // per wavefront slot (nwave slots, cleared by each fold) and their

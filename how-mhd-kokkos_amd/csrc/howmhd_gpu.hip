@@ -239,7 +239,6 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     }
     ok = ok && alloc((void**)&c->dmax, 4 * sizeof(unsigned long long));
     ok = ok && alloc((void**)&c->cflpart, 3 * (size_t)blocks_for(c->ncell, 256) * sizeof(double));
-    if (p->precision == HOWMHD_PREC_FP32) ok = ok && alloc((void**)&c->prim, 8 * c->ncell * sizeof(float));
     ok = ok && alloc((void**)&c->mlp, sizeof(MlpDev));
     // one counter slot per wavefront of the largest sweep grid
     for (int d = 0; d < 3; ++d) {
@@ -275,7 +274,6 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
         if (c->ct[t]) (void)hipFree(c->ct[t]);
     if (c->dmax) (void)hipFree(c->dmax);
     if (c->cflpart) (void)hipFree(c->cflpart);
-    if (c->prim) (void)hipFree(c->prim);
     if (c->mlp) (void)hipFree(c->mlp);
     if (c->istats) (void)hipFree(c->istats);
     if (c->iwave) (void)hipFree(c->iwave);
@@ -294,11 +292,6 @@ int howmhd_gpu_upload_q0(howmhd_gpu_ctx* c, const double* q0) {
     if (!c || !q0) return HOWMHD_EINVAL;
     const int rc = upload_ref(c, q0, c->qbuf[c->qid[0]], 8);
     if (rc) return rc;
-    if (c->prim) {
-        hipLaunchKernelGGL(k_prim, dim3(blocks_for(c->ncell, 256)), dim3(256), 0, c->stream,
-                           (const double*)c->qbuf[c->qid[0]], c->prim, c->g, c->ph);
-        HCHECK(c, hipGetLastError());
-    }
     c->faces_ready = false;
     c->cfl_fresh = false;
     return HOWMHD_OK;

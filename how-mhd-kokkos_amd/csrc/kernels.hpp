@@ -111,10 +111,6 @@ enum SweepMode { SW_PLAIN = 0, SW_COLLECT = 1, SW_ROUTED = 2 };
 template <typename R>
 struct SweepArgs {
     const double* q;    // source stage, 8 arrays of ncell
-    // FP32 sweeps read instead the primitives of the source stage that the
-    // stage combination (or k_prim after an upload) wrote: 8 arrays of ncell,
-    // (rho, vx, vy, vz, Bx, By, Bz, p)
-    const float* prim;
     R* dF;              // 7 x n_if planes (see dF_index)
     // constrained transport active: components 4, 5 of dF (the
     // transverse-B fluxes) feed only the CT terms, not k_combine, and are
@@ -384,20 +380,14 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
         return ((long long)cz * g.n[1] + cy) * g.n[0] + cx;
     };
     constexpr int NIT = (T::CELLS + 255) / 256;
-    constexpr bool kPrim = sizeof(R) == 4;  // FP32 form: from the stage's primitives
-    using QT = typename std::conditional<kPrim, float, double>::type;
-    QT qs[NIT][8];
+    double qs[NIT][8];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int s = tid + 256 * it;
         if (s < T::CELLS) {
             const long long idx = cell_index(s);
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const long long o = (long long)kPerm[DIR][m] * g.ncell + idx;
-                if constexpr (kPrim) qs[it][m] = A.prim[o];
-                else qs[it][m] = A.q[o];
-            }
+            for (int m = 0; m < 8; ++m) qs[it][m] = A.q[(long long)kPerm[DIR][m] * g.ncell + idx];
         }
     }
     bool cell_bad = false;
@@ -406,8 +396,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
         const int s = tid + 256 * it;
         if (s < T::CELLS) {
             R out[C_NFIELD];
-            if constexpr (kPrim) cell_values_prim(qs[it], (float)A.ph.gam, out);
-            else cell_values<R, DM>(qs[it], A.ph.gam, A.ph.rhomin, A.ph.pgmin, out, &cell_bad);
+            cell_values<R, DM>(qs[it], A.ph.gam, A.ph.rhomin, A.ph.pgmin, out, &cell_bad);
 #pragma unroll
             for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
 #pragma unroll
@@ -1175,9 +1164,6 @@ struct CombineArgs {
     // stage 5: per-block CFL maxima of the new q(0) (3 doubles per block),
     // which the next tstep reduces instead of reading q(0) again; else null
     double* cfl;
-    // FP32 form: the primitives of this stage for the next stage's sweeps
-    // (FP32 roundings of prot's rho, v, B and p); unused by the FP64 forms
-    float* prim;
 };
 
 // 6th-order face -> centre interpolation of src/fluxct.cpp:891-896:
@@ -1293,12 +1279,6 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
         v[0] = rho;
 #pragma unroll
         for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
-        if constexpr (sizeof(R) == 4) {
-            const float pv[8] = {(float)rho, (float)vx, (float)vy, (float)vz,
-                                 (float)v[4], (float)v[5], (float)v[6], (float)pg};
-#pragma unroll
-            for (int m = 0; m < 8; ++m) A.prim[(long long)m * g.ncell + i] = pv[m];
-        }
         if (A.cfl) cfl_cell(v, A.ph, m0, m1, m2);
     }
     if (A.cfl) block_max3(m0, m1, m2, A.cfl + 3 * (long long)blockIdx.x);

@@ -264,46 +264,6 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     (void)rrho;
 }
 
-// FP32 form: the same per-cell values from the primitives that the stage
-// combination wrote (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p in sweep order; FP32
-// roundings of prot's FP64 values, after its floors).  E -> p, the step that
-// cancels at low beta, stays in the combination's FP64; here E, M and H are
-// formed in FP32 from sums and products of the primitives.
-__device__ __forceinline__ void cell_values_prim(const float* ps, float gam, float* out) {
-    const float rho = ps[0], vx = ps[1], vy = ps[2], vz = ps[3];
-    const float Bx = ps[4], By = ps[5], Bz = ps[6], pg = ps[7];
-    const float Mx = rho * vx, My = rho * vy, Mz = rho * vz;
-    const float vv2 = vx * vx + vy * vy + vz * vz;
-    const float BB2 = Bx * Bx + By * By + Bz * Bz;
-    const float EE = pg / (gam - 1.0f) + 0.5f * (rho * vv2 + BB2);
-    const float irho = rcp_fast(rho);
-    const float HH = (EE + pg) * irho;
-    const float pt = pg + 0.5f * BB2;
-    const float vdB = vx * Bx + vy * By + vz * Bz;
-    // speeds as in cell_values' FP32 branch
-    const float bnx2 = Bx * Bx * irho;
-    const float btr2 = (By * By + Bz * Bz) * irho;
-    const float bbn2 = bnx2 + btr2;
-    const float cs2 = rmax(0.0f, gam * rabs(pg * irho));
-    const float dd = bbn2 - cs2;
-    const float root = rsqrt_(dd * dd + 4.0f * btr2 * cs2);
-    const float lf2 = 0.5f * (bbn2 + cs2 + root);
-    const float lf = rsqrt_(lf2);
-    const float la = rsqrt_(bnx2);
-    const float ls = (lf2 > 0.0f) ? rsqrt_(bnx2 * cs2 * rcp_fast(lf2)) : 0.0f;
-    out[C_RHO] = rho;  out[C_MN] = Mx;  out[C_MT1] = My;  out[C_MT2] = Mz;
-    out[C_BT1] = By;   out[C_BT2] = Bz; out[C_E] = EE;
-    out[C_F0] = rho * vx;
-    out[C_F1] = Mx * vx + pt - Bx * Bx;
-    out[C_F2] = My * vx - Bx * By;
-    out[C_F3] = Mz * vx - Bx * Bz;
-    out[C_F4] = By * vx - Bx * vy;
-    out[C_F5] = Bz * vx - Bx * vz;
-    out[C_F6] = (EE + pt) * vx - Bx * vdB;
-    out[C_VN] = vx;  out[C_LF] = lf;  out[C_LA] = la;  out[C_LS] = ls;
-    out[C_VT1] = vy; out[C_VT2] = vz; out[C_BN] = Bx; out[C_P] = pg; out[C_H] = HH;
-}
-
 // ------------------------------------------------------ interface eigensystem
 // Scalars of the interface eigensystem (src/ssprk.cpp:610-659).  Rows of L
 // and columns of R are built from them one wave at a time.

@@ -62,7 +62,6 @@ template <typename R>
 SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     SweepArgs<R> a{};
     a.q = qsrc;
-    a.prim = c->prim;
     a.dF = (R*)c->dF[d];
     // x: fsy, fsz; y: gsz, gsx; z: hsx, hsy
     a.ct_a = (R*)c->ct[2 * d];
@@ -191,7 +190,6 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     ca.s = s;
     ca.ph = c->ph;
     ca.cfl = (iw == 5) ? c->cflpart : nullptr;
-    ca.prim = c->prim;
     {
         TimedLaunch tl(c, HOWMHD_K_COMBINE);
         hipLaunchKernelGGL((k_combine<R, EX>), dim3(nb), dim3(256), 0, c->stream, ca);

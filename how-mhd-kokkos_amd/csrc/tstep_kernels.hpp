@@ -93,26 +93,6 @@ __global__ __launch_bounds__(256) void k_init_faces(const double* q, double* b, 
                           Bz[lin(g, x, y, bmap(z + 2, nz, pz))]) / 16.0;
 }
 
-// FP32 form: the primitives of an uploaded q(0) for the first stage's sweeps,
-// as the stage combination's prot forms them (src/prot.cpp:10-51): rho and p
-// floored, v = M / rho, p from E in FP64; stored as FP32.
-__global__ __launch_bounds__(256) void k_prim(const double* q, float* prim, Geom g, Phys ph) {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= g.ncell) return;
-    double v[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = q[m * g.ncell + i];
-    const double gam = ph.gam;
-    const double rho = fmax(v[0], ph.rhomin);
-    const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
-    const double vv2 = vx * vx + vy * vy + vz * vz;
-    const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
-    const double pg = fmax((gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2)), ph.pgmin);
-    const float pv[8] = {(float)rho, (float)vx, (float)vy, (float)vz, (float)v[4], (float)v[5], (float)v[6], (float)pg};
-#pragma unroll
-    for (int m = 0; m < 8; ++m) prim[m * g.ncell + i] = pv[m];
-}
-
 // ------------------------------------------- host layout <-> device layout
 // One variable in the reference's host layout: (nx+6) x (ny+6) x (nz+6)
 // values, three ghost layers per side, z fastest (the q(m, i, j, k) view of
