@@ -11,6 +11,21 @@
  * deliberate choice: the reference accumulates dF with atomic_add
  * (src/ssprk.cpp:941-943), whose order is not fixed under OpenMP; here the
  * waves are added in the order m = 0..6, which is the Serial-backend order.
+ *
+ * How each piece is pinned (tests/test_oracle.py, test_indicator_oracle.py):
+ * WENO5 + LLF against textbook Jiang-Shu WENO5; the eigensystem by R being
+ * the eigenvectors of the flux Jacobian and L R = I; the sweeps by constant
+ * states, axis relabelling and conservation; CT by the preserved D4
+ * divergence; the whole step by convergence orders; tstep by its closed form
+ * on a uniform state; the floors by their bounds.  The reference ships no
+ * golden data; tests/golden holds regression fixtures of this file.
+ *
+ * Parity unpinned (only the reference text fixes them; this file reproduces
+ * them by reading it): the operation order inside each formula, the -0.0
+ * weights of the CT flux terms, the stage-5 k4/k1 and k5/k1 divisions, prot
+ * always rebuilding E, floors written back to the workspace only, and the
+ * padded wrap of bc_index_ct (it only touches ghost results that are
+ * refilled).
  */
 #include "howmhd_oracle.h"
 
