@@ -13,8 +13,10 @@ as in the reference driver.
 value = 256^3 * steps / wall time of the timed steps.  After the timed loop a
 short run with per-kernel HIP-event timing on the library's stream gives the
 roofline entry of the dominant kernel.  The CPU baseline is the C restatement
-of the reference time step (oracle/) on the host cores, on a bounded sample
-(the benchmark workload, one step at 256^3).
+of the reference time step (oracle/) on the host's physical cores with
+OMP_PROC_BIND=spread and OMP_PLACES=cores (the reference README's settings),
+in a child process, on a bounded sample (the benchmark workload, two steps at
+256^3).
 
 Prints ONE JSON line.
 """
@@ -65,7 +67,7 @@ def parse_args():
     ap.add_argument("--indicator", action="store_true",
                     help="time the form with the learned indicator (weights/indicator_mlp.txt) instead")
     ap.add_argument("--cpu-size", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--no-forms", action="store_true", help="skip the FP32 form leg")
@@ -75,52 +77,121 @@ def parse_args():
     return ap.parse_args()
 
 
-def traffic_from_profile(kernel_key: str):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_mhd_blast3d_256_fp64.txt"
+
+
+def traffic_table() -> dict:
+    """HBM bytes per launch of each kernel from the committed rocprofv3 PMC
     summary of the benchmark workload (profiles/traffic_mhd_blast3d_256_fp64.txt:
-    '<kernel> <bytes_per_launch>' lines), or None."""
-    f = ROOT / "profiles" / "traffic_mhd_blast3d_256_fp64.txt"
-    if not f.exists():
-        return None
-    for line in f.read_text().splitlines():
+    '<kernel> <bytes_per_launch>' lines); empty when it is absent."""
+    out = {}
+    if not TRAFFIC_FILE.exists():
+        return out
+    for line in TRAFFIC_FILE.read_text().splitlines():
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel_key and not line.startswith("#"):
+        if len(parts) >= 2 and not line.startswith("#"):
             try:
-                return float(parts[1])
+                out[parts[0]] = float(parts[1])
             except ValueError:
-                return None
-    return None
+                pass
+    return out
+
+
+def traffic_per_step(tab: dict, launches_per_step: dict):
+    """Counter bytes of one step: per-launch bytes x launches per step, or
+    None when a kernel of the step is missing from the table."""
+    tot = 0.0
+    for k, n in launches_per_step.items():
+        if k not in tab:
+            return None
+        tot += tab[k] * n
+    return tot
+
+
+CPU_CHILD = r"""
+import json, sys, time
+sys.path[:0] = [sys.argv[1], sys.argv[2]]
+import howmhd as H
+import oracle_ctypes as O
+n, steps, threads = int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5])
+O.set_threads(threads)
+run = H.parse(H.make_input(problem="mhd_blast3d", nx=n, ny=n, nz=n, nt=1, tend=1.0, cour=0.4,
+                           gam="1.6666666666666667"))
+q0 = H.init_problem(run)
+ref = O.RefSolver(H.gpu_params(run))
+ref.upload_q0(q0)
+ref.init_faces()
+del q0
+t0 = time.perf_counter()
+for _ in range(steps):
+    ref.ssprk(ref.tstep())
+print(json.dumps({"seconds": time.perf_counter() - t0}))
+"""
+
+
+def host_cpu():
+    """(CPU model, physical cores of the machine, physical cores among the
+    CPUs this process may run on) from lscpu."""
+    import subprocess
+    model, phys_all, phys_mine = "unknown", None, None
+    try:
+        info = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in info.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+        rows = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                              timeout=30).stdout.splitlines()
+        allowed = os.sched_getaffinity(0)
+        cores, mine = set(), set()
+        for r in rows:
+            if r.startswith("#"):
+                continue
+            cpu, core, sock = (int(x) if x else 0 for x in r.split(","))
+            cores.add((sock, core))
+            if cpu in allowed:
+                mine.add((sock, core))
+        phys_all, phys_mine = len(cores), len(mine)
+    except Exception:
+        pass
+    return model, phys_all, phys_mine
 
 
 def cpu_baseline(n: int, steps: int):
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import howmhd as H
-    import oracle_ctypes as O
-
-    threads = min(16, len(os.sched_getaffinity(0)))
-    O.set_threads(threads)
-    run = H.parse(H.make_input(problem="mhd_blast3d", nx=n, ny=n, nz=n, nt=1, tend=1.0, cour=0.4,
-                               gam="1.6666666666666667"))
-    q0 = H.init_problem(run)
-    ref = O.RefSolver(H.gpu_params(run))
-    ref.upload_q0(q0)
-    ref.init_faces()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ref.ssprk(ref.tstep())
-    el = time.perf_counter() - t0
-    ref.close()
+    """The oracle on the host's physical cores, OMP_PROC_BIND=spread,
+    OMP_PLACES=cores (README.md:409-412 of the reference), in a child process
+    so that the OpenMP runtime starts with those settings.  The thread count
+    is the physical cores among the CPUs this process may use, capped by
+    OMP_NUM_THREADS when the environment sets it (a machine that shares its
+    cores among GPUs gives each its share that way)."""
+    import subprocess
+    model, phys_all, phys_mine = host_cpu()
+    threads = phys_mine or len(os.sched_getaffinity(0))
+    env_cap = os.environ.get("OMP_NUM_THREADS")
+    if env_cap and env_cap.isdigit() and int(env_cap) > 0:
+        threads = min(threads, int(env_cap))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+    res = subprocess.run([sys.executable, "-c", CPU_CHILD, str(ROOT / "how-mhd-kokkos_amd"), str(ROOT / "oracle"),
+                          str(n), str(steps), str(threads)], env=env, capture_output=True, text=True, timeout=900)
+    if res.returncode != 0:
+        raise RuntimeError(f"CPU baseline child failed ({res.returncode}): {res.stderr[-2000:]}")
+    el = json.loads(res.stdout.strip().splitlines()[-1])["seconds"]
     return {
         "value": n ** 3 * steps / el,
         "unit": "zone-cycles/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"mhd_blast3d {n}^3, {steps} step(s) (tstep + SSPRK(5,4)) of the C restatement of the "
-                  f"reference time step, OpenMP, {threads} threads; {el:.2f} s",
+        "cpu_model": model,
+        "physical_cores_machine": phys_all,
+        "physical_cores_allowed": phys_mine,
+        "omp": "OMP_PROC_BIND=spread OMP_PLACES=cores",
+        "sample": f"mhd_blast3d {n}^3, {steps} steps (tstep + SSPRK(5,4)) of the C restatement of the "
+                  f"reference time step, OpenMP, {threads} threads on {model}; {el:.2f} s",
     }
 
 
 WEIGHTS = ROOT / "how-mhd-kokkos_amd" / "weights" / "indicator_mlp.txt"
+# conserved variables in the reference order (README.md:53-64)
+VARS = ["rho", "Mx", "My", "Mz", "Bx", "By", "Bz", "E"]
 
 
 def form_leg(H, torch, run, args, fp64_value, precision, indicator=False):
@@ -188,6 +259,8 @@ def form_leg(H, torch, run, args, fp64_value, precision, indicator=False):
         "rel_l1_rho": float(l1[0]),
         "rel_l1_E": float(l1[7]),
         "rel_l1_max": float(np.max(l1)),
+        "rel_l1_max_component": VARS[int(np.argmax(l1))],
+        "rel_l1_per_component": {VARS[m]: float(l1[m]) for m in range(8)},
         "magnetic_energy_rel_err": abs(emag(af) - emag(a64)) / emag(a64),
         "divb_ct_max": df.max_divb_ct,
     }
@@ -275,19 +348,25 @@ def main():
         kernels[k] = ent
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dom]
+    tab = traffic_table() if (plain64 and n == 256) else {}
     if dom.startswith("sweep"):
         achieved = dk["achieved_tflops"]
-        roof = {"bound": "mfma", "achieved": achieved, "peak": flop_peak, "unit": "TFLOP/s",
+        roof = {"bound": "valu", "achieved": achieved, "peak": flop_peak, "unit": "TFLOP/s",
                 "frac": round(achieved / flop_peak, 4),
-                "traffic": traffic_from_profile(dom) if plain64 else None,
-                "kernel": f"k_sweep ({dom}); FP64 vector-ALU bound (no MFMA used: MI355X FP64 vector peak = "
-                          f"FP64 matrix peak = 78.6 TFLOP/s)",
+                "traffic": tab.get(dom),
+                "kernel": f"k_sweep ({dom}); FP64 vector-ALU bound (no MFMA: the sweep has no dense "
+                          f"contraction; MI355X FP64 vector peak = 78.6 TFLOP/s)",
                 "per_launch": f"{FLOP_SWEEP_CELL:.0f} FLOP/cell x {zones} cells"}
     else:
         achieved = dk["achieved_gbs"]
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(dom) if plain64 else None,
-                "kernel": dom}
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": tab.get(dom), "kernel": dom}
+    # counter bytes of a whole step against its compulsory bytes (1,472 B per
+    # zone-step): 5 launches per stage kernel, one CFL fold
+    step_bytes = traffic_per_step(tab, {"sweep_x": 5, "sweep_y": 5, "sweep_z": 5, "emf": 5, "faces": 5,
+                                        "combine": 5, "cfl": 1}) if tab else None
+    roof["traffic_step"] = step_bytes
+    roof["traffic_ratio"] = round(step_bytes / (BYTES_ZONE_STEP * zones), 3) if step_bytes else None
     whole = {
         "fp64_tflops_effective": round(FLOP_ZONE_STEP * value / 1e12, 3),
         "frac_fp64_roof": round(FLOP_ZONE_STEP * value / (PEAK_FP64_TFLOPS * 1e12), 4),

@@ -38,6 +38,17 @@ struct howmhd_gpu_ctx {
     // k_combine); cfl_fresh: they belong to the current q(0)
     double* cflpart = nullptr;
     bool cfl_fresh = false;
+    // cells whose density / pressure floor the stage-5 prot applied, per
+    // block of the last step's stage-5 combination (2 per block);
+    // floors_fresh: they belong to the current q(0)
+    unsigned* floorpart = nullptr;
+    bool floors_fresh = false;
+    // diagnostics: per-block partials of k_diag and the folded values
+    double* diagpart = nullptr;
+    double* diagout = nullptr;
+    // host transfers: one variable in the reference layout
+    // ((nx+6)(ny+6)(nz+6) doubles), allocated with the context
+    double* stage_ref = nullptr;
     // learned indicator: device weights, routing mode (0 off, 1 on with the
     // plain-WENO fallback, 2 on for every step), the routed sweeps' counters
     // per wavefront slot (nwave slots, cleared by each fold) and their
@@ -54,6 +65,9 @@ struct howmhd_gpu_ctx {
     unsigned* fixlist = nullptr;
     unsigned* fixctl = nullptr;
     unsigned long long* istats = nullptr;
+    // (flagged, total) of the most recent routed step, and whether one ran
+    unsigned long long* ilast = nullptr;
+    bool any_routed = false;
     // mode 1: counts of the last routed step travel to pinned host memory
     // behind an event; a step whose wavefronts were mostly flagged sends the
     // following steps to the plain sweep

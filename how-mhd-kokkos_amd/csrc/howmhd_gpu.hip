@@ -24,7 +24,6 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
-#include <vector>
 
 #if !HOWMHD_EXACT
 #error "howmhd_gpu.hip is built with HOWMHD_EXACT=1"
@@ -42,6 +41,9 @@ std::string g_create_error = "no error";
 // which one step routes again.
 constexpr double kFallbackWaveFraction = 0.9;
 constexpr int kFallbackSteps = 8;
+
+// What howmhd_gpu_build_kind returns: the target and the rounding of the forms.
+constexpr const char* kBuildKind = "gfx950: fp64 reference rounding; fp64fast and fp32 with FMA contraction";
 
 // Folds the counter copy of the last routed step into the running totals
 // (wait: block until it has arrived; else only if it has) and takes the
@@ -95,25 +97,12 @@ uint16_t bf16_bits(float f) {
     return (uint16_t)(u >> 16);
 }
 
-int host_map(int a, int n, int per) {
-    if (n == 1) return 0;
-    if (per) return ((a % n) + n) % n;
-    return a < 0 ? 0 : (a >= n ? n - 1 : a);
-}
-
 // Transfers between the caller's arrays in the reference layout
 // (m, i, j, k), three ghost layers per side, and the device layout
-// (m, z, y, x) of active cells.  One variable at a time goes through a device
-// buffer of one variable's reference-layout size, allocated for the call: the
-// copy reads or writes the caller's (pageable) memory directly and the
-// transpose runs on the device (k_ref_to_dev, k_dev_to_ref).
-struct StageBuf {
-    double* p = nullptr;
-    ~StageBuf() {
-        if (p) (void)hipFree(p);
-    }
-};
-
+// (m, z, y, x) of active cells.  One variable at a time goes through the
+// context's staging buffer of one variable's reference-layout size: the copy
+// reads or writes the caller's (pageable) memory directly and the transpose
+// runs on the device (k_ref_to_dev, k_dev_to_ref).
 size_t ref_size(const howmhd_gpu_ctx* c) {
     return ((size_t)c->g.n[0] + 6) * ((size_t)c->g.n[1] + 6) * ((size_t)c->g.n[2] + 6);
 }
@@ -122,12 +111,11 @@ unsigned lay_planes(int n) { return (unsigned)std::min(n, 65535); }
 
 int upload_ref(howmhd_gpu_ctx* c, const double* ref, double* dev, int nvar) {
     const size_t n6 = ref_size(c);
-    StageBuf st;
-    HCHECK(c, hipMalloc(&st.p, n6 * sizeof(double)));
     const dim3 grid(blocks_for(c->g.n[2], LAY_T), blocks_for(c->g.n[0], LAY_T), lay_planes(c->g.n[1]));
     for (int m = 0; m < nvar; ++m) {
-        HCHECK(c, hipMemcpyAsync(st.p, ref + (size_t)m * n6, n6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_ref_to_dev, grid, dim3(LAY_T, 8), 0, c->stream, (const double*)st.p,
+        HCHECK(c, hipMemcpyAsync(c->stage_ref, ref + (size_t)m * n6, n6 * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        hipLaunchKernelGGL(k_ref_to_dev, grid, dim3(LAY_T, 8), 0, c->stream, (const double*)c->stage_ref,
                            dev + (size_t)m * c->ncell, c->g);
         HCHECK(c, hipGetLastError());
     }
@@ -137,16 +125,54 @@ int upload_ref(howmhd_gpu_ctx* c, const double* ref, double* dev, int nvar) {
 
 int download_ref(howmhd_gpu_ctx* c, const double* dev, double* ref, int nvar) {
     const size_t n6 = ref_size(c);
-    StageBuf st;
-    HCHECK(c, hipMalloc(&st.p, n6 * sizeof(double)));
     const dim3 grid(blocks_for(c->g.n[2] + 6, LAY_T), blocks_for(c->g.n[0] + 6, LAY_T), lay_planes(c->g.n[1] + 6));
     for (int m = 0; m < nvar; ++m) {
-        hipLaunchKernelGGL(k_dev_to_ref, grid, dim3(LAY_T, 8), 0, c->stream, dev + (size_t)m * c->ncell, st.p, c->g);
+        hipLaunchKernelGGL(k_dev_to_ref, grid, dim3(LAY_T, 8), 0, c->stream, dev + (size_t)m * c->ncell,
+                           c->stage_ref, c->g);
         HCHECK(c, hipGetLastError());
-        HCHECK(c, hipMemcpyAsync(ref + (size_t)m * n6, st.p, n6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HCHECK(c, hipMemcpyAsync(ref + (size_t)m * n6, c->stage_ref, n6 * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
     }
     HCHECK(c, hipStreamSynchronize(c->stream));
     return HOWMHD_OK;
+}
+
+// dt of src/tstep.cpp:123-135 from the per-direction maxima
+double dt_from_vmax(const howmhd_params& p, const double* vmax) {
+    const double tiny = 1.0e-300;
+    double invdt = 0.0;
+    if (p.nx > 1) invdt += std::fmax(vmax[0], tiny) / p.dx;
+    if (p.ny > 1) invdt += std::fmax(vmax[1], tiny) / p.dy;
+    if (p.nz > 1) invdt += std::fmax(vmax[2], tiny) / p.dz;
+    if (invdt <= 0.0) invdt = std::fmax(vmax[0], tiny) / p.dx;
+    return p.cour / invdt;
+}
+
+// Largest axis length the launch grids accept on the y and z axes: the
+// sweeps, the face update and the EMF build put an axis length (or a tile
+// count of it) into gridDim.y or gridDim.z.
+constexpr int kMaxYZ = 65535;
+
+// Parameter checks of howmhd_gpu_create (no HIP call); "" when valid.
+std::string check_params(const howmhd_params* p) {
+    const int n[3] = {p->nx, p->ny, p->nz};
+    for (int d = 0; d < 3; ++d) {
+        if (n[d] < 1 || n[d] == 2) return "howmhd_gpu_create: each axis needs n == 1 or n >= 3 active cells";
+        if (p->bc[d] != HOWMHD_BC_PERIODIC && p->bc[d] != HOWMHD_BC_OPEN)
+            return "howmhd_gpu_create: unknown boundary condition code";
+    }
+    if (p->ny > kMaxYZ || p->nz > kMaxYZ)
+        return "howmhd_gpu_create: ny and nz must not exceed " + std::to_string(kMaxYZ) +
+               " (launch grid limit on the y and z axes); got ny = " + std::to_string(p->ny) +
+               ", nz = " + std::to_string(p->nz);
+    if ((long long)p->nx * p->ny * p->nz > (1LL << 40)) return "howmhd_gpu_create: more than 2^40 active cells";
+    if (p->precision != HOWMHD_PREC_FP64 && p->precision != HOWMHD_PREC_FP32 && p->precision != HOWMHD_PREC_FP64_FAST)
+        return "howmhd_gpu_create: unknown precision";
+    if (p->indicator != HOWMHD_INDICATOR_OFF && p->indicator != HOWMHD_INDICATOR_MLP)
+        return "howmhd_gpu_create: unknown indicator";
+    if (!(p->dx > 0 && p->dy > 0 && p->dz > 0) || !(p->gam > 1.0))
+        return "howmhd_gpu_create: dx, dy, dz must be > 0 and gam > 1";
+    return "";
 }
 
 }  // namespace
@@ -159,30 +185,12 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
         return HOWMHD_EINVAL;
     }
     *out = nullptr;
+    const std::string bad = check_params(p);
+    if (!bad.empty()) {
+        g_create_error = bad;
+        return HOWMHD_EINVAL;
+    }
     const int n[3] = {p->nx, p->ny, p->nz};
-    for (int d = 0; d < 3; ++d) {
-        if (n[d] < 1 || n[d] == 2) {
-            g_create_error = "howmhd_gpu_create: each axis needs n == 1 or n >= 3 active cells";
-            return HOWMHD_EINVAL;
-        }
-        if (p->bc[d] != HOWMHD_BC_PERIODIC && p->bc[d] != HOWMHD_BC_OPEN) {
-            g_create_error = "howmhd_gpu_create: unknown boundary condition code";
-            return HOWMHD_EINVAL;
-        }
-    }
-    if (p->precision != HOWMHD_PREC_FP64 && p->precision != HOWMHD_PREC_FP32 &&
-        p->precision != HOWMHD_PREC_FP64_FAST) {
-        g_create_error = "howmhd_gpu_create: unknown precision";
-        return HOWMHD_EINVAL;
-    }
-    if (p->indicator != HOWMHD_INDICATOR_OFF && p->indicator != HOWMHD_INDICATOR_MLP) {
-        g_create_error = "howmhd_gpu_create: unknown indicator";
-        return HOWMHD_EINVAL;
-    }
-    if (!(p->dx > 0 && p->dy > 0 && p->dz > 0) || !(p->gam > 1.0)) {
-        g_create_error = "howmhd_gpu_create: dx, dy, dz must be > 0 and gam > 1";
-        return HOWMHD_EINVAL;
-    }
     howmhd_gpu_ctx* c = new howmhd_gpu_ctx();
     c->p = *p;
     for (int d = 0; d < 3; ++d) {
@@ -239,6 +247,10 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     }
     ok = ok && alloc((void**)&c->dmax, 4 * sizeof(unsigned long long));
     ok = ok && alloc((void**)&c->cflpart, 3 * (size_t)blocks_for(c->ncell, 256) * sizeof(double));
+    ok = ok && alloc((void**)&c->floorpart, 2 * (size_t)blocks_for(c->ncell, 256) * sizeof(unsigned));
+    ok = ok && alloc((void**)&c->diagpart, (size_t)DIAG_BLOCKS * DIAG_NV * sizeof(double));
+    ok = ok && alloc((void**)&c->diagout, DIAG_NV * sizeof(double));
+    ok = ok && alloc((void**)&c->stage_ref, ref_size(c) * sizeof(double));
     ok = ok && alloc((void**)&c->mlp, sizeof(MlpDev));
     // one counter slot per wavefront of the largest sweep grid
     for (int d = 0; d < 3; ++d) {
@@ -250,6 +262,7 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     ok = ok && alloc((void**)&c->fixlist, (size_t)(c->nwave / 4) * sizeof(unsigned));
     ok = ok && alloc((void**)&c->fixctl, 2 * sizeof(unsigned));
     ok = ok && alloc((void**)&c->istats, 4 * sizeof(unsigned long long));
+    ok = ok && alloc((void**)&c->ilast, 2 * sizeof(unsigned long long));
     ok = ok && hipHostMalloc((void**)&c->istats_host, 4 * sizeof(unsigned long long), hipHostMallocDefault) ==
                    hipSuccess;
     ok = ok && hipEventCreateWithFlags(&c->istats_ev, hipEventDisableTiming) == hipSuccess;
@@ -274,6 +287,11 @@ int howmhd_gpu_destroy(howmhd_gpu_ctx* c) {
         if (c->ct[t]) (void)hipFree(c->ct[t]);
     if (c->dmax) (void)hipFree(c->dmax);
     if (c->cflpart) (void)hipFree(c->cflpart);
+    if (c->floorpart) (void)hipFree(c->floorpart);
+    if (c->diagpart) (void)hipFree(c->diagpart);
+    if (c->diagout) (void)hipFree(c->diagout);
+    if (c->stage_ref) (void)hipFree(c->stage_ref);
+    if (c->ilast) (void)hipFree(c->ilast);
     if (c->mlp) (void)hipFree(c->mlp);
     if (c->istats) (void)hipFree(c->istats);
     if (c->iwave) (void)hipFree(c->iwave);
@@ -294,6 +312,7 @@ int howmhd_gpu_upload_q0(howmhd_gpu_ctx* c, const double* q0) {
     if (rc) return rc;
     c->faces_ready = false;
     c->cfl_fresh = false;
+    c->floors_fresh = false;
     return HOWMHD_OK;
 }
 
@@ -338,14 +357,7 @@ int howmhd_gpu_tstep(howmhd_gpu_ctx* c, double* dt) {
     HCHECK(c, hipStreamSynchronize(c->stream));
     double vmax[3];
     for (int d = 0; d < 3; ++d) std::memcpy(&vmax[d], &h[d], sizeof(double));
-    // src/tstep.cpp:123-135
-    const double tiny = 1.0e-300;
-    double invdt = 0.0;
-    if (c->p.nx > 1) invdt += std::fmax(vmax[0], tiny) / c->p.dx;
-    if (c->p.ny > 1) invdt += std::fmax(vmax[1], tiny) / c->p.dy;
-    if (c->p.nz > 1) invdt += std::fmax(vmax[2], tiny) / c->p.dz;
-    if (invdt <= 0.0) invdt = std::fmax(vmax[0], tiny) / c->p.dx;
-    *dt = c->p.cour / invdt;
+    *dt = dt_from_vmax(c->p, vmax);
     return HOWMHD_OK;
 }
 
@@ -381,10 +393,15 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
         if (rc) return rc;
     }
     if (c->route_now) {
-        // the step's per-wavefront counters into the totals (slots cleared)
-        const unsigned nb = std::min(1024u, blocks_for((size_t)c->nwave, 256));
-        hipLaunchKernelGGL(k_fold_wave_stats, dim3(nb), dim3(256), 0, c->stream, c->iwave, c->nwave, c->istats);
+        // the step's per-wavefront counters into the totals and into the
+        // step's own (flagged, total) (slots cleared)
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->ilast, 2);
         HCHECK(c, hipGetLastError());
+        const unsigned nb = std::min(1024u, blocks_for((size_t)c->nwave, 256));
+        hipLaunchKernelGGL(k_fold_wave_stats, dim3(nb), dim3(256), 0, c->stream, c->iwave, c->nwave, c->istats,
+                           c->ilast);
+        HCHECK(c, hipGetLastError());
+        c->any_routed = true;
     }
     if (c->ind_mode == 1 && c->route_now && !c->istats_pending) {
         // this step's totals to the host, then clear them for the next
@@ -399,7 +416,8 @@ int howmhd_gpu_ssprk(howmhd_gpu_ctx* c, double dt) {
     // q(0) <- q(5), faces likewise: exchange the buffers of stages 0 and 5
     std::swap(c->qid[0], c->qid[1]);
     c->qid[5] = c->qid[1];
-    c->cfl_fresh = true;  // the stage-5 combination left the CFL maxima of the new q(0)
+    c->cfl_fresh = true;     // the stage-5 combination left the CFL maxima of the new q(0)
+    c->floors_fresh = true;  // and the counts of the floors its prot applied
     return HOWMHD_OK;
 }
 
@@ -416,69 +434,35 @@ int howmhd_gpu_download_faces(howmhd_gpu_ctx* c, double* b) {
 int howmhd_gpu_diag(howmhd_gpu_ctx* c, howmhd_diag* d) {
     if (!c || !d) return HOWMHD_EINVAL;
     std::memset(d, 0, sizeof(*d));
-    const size_t nc = c->ncell;
-    std::vector<double> q(8 * nc), b(3 * nc);
-    HCHECK(c, hipMemcpyAsync(q.data(), c->qbuf[c->qid[0]], q.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HCHECK(c, hipMemcpyAsync(b.data(), c->fbuf[c->qid[0]], b.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DiagArgs a;
+    a.q = c->qbuf[c->qid[0]];
+    a.b = c->fbuf[c->qid[0]];
+    a.g = c->g;
+    a.gam = c->p.gam;
+    a.dx = c->p.dx;
+    a.dy = c->p.dy;
+    a.dz = c->p.dz;
+    a.floors = c->floors_fresh ? c->floorpart : nullptr;
+    a.nfloor = blocks_for(c->ncell, 256);
+    a.part = c->diagpart;
+    const int nb = (int)std::min<size_t>(DIAG_BLOCKS, blocks_for(c->ncell, 256));
+    hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, c->stream, a);
+    HCHECK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_diag_fold, dim3(1), dim3(256), 0, c->stream, (const double*)c->diagpart, nb, c->diagout);
+    HCHECK(c, hipGetLastError());
+    double h[DIAG_NV];
+    unsigned long long il[2] = {0ull, 0ull};
+    HCHECK(c, hipMemcpyAsync(h, c->diagout, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    if (c->any_routed) HCHECK(c, hipMemcpyAsync(il, c->ilast, sizeof(il), hipMemcpyDeviceToHost, c->stream));
     HCHECK(c, hipStreamSynchronize(c->stream));
-    const int nx = c->g.n[0], ny = c->g.n[1], nz = c->g.n[2];
-    const double gam = c->p.gam;
-    d->min_rho = 1e300;
-    d->min_p = 1e300;
-    for (int m = 0; m < 8; ++m) {
-        double s = 0.0;
-        for (size_t i = 0; i < nc; ++i) s += q[m * nc + i];
-        d->sum_cons[m] = s;
-    }
-    auto M = [&](int a, int n, int per) { return host_map(a, n, per); };
-    const double aa = 9.0 / 8.0, bb = -1.0 / 24.0;
-    for (int z = 0; z < nz; ++z)
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) {
-                const size_t i = ((size_t)z * ny + y) * nx + x;
-                const double rho = q[i];
-                const double vx = q[nc + i] / rho, vy = q[2 * nc + i] / rho, vz = q[3 * nc + i] / rho;
-                const double BB2 = q[4 * nc + i] * q[4 * nc + i] + q[5 * nc + i] * q[5 * nc + i] + q[6 * nc + i] * q[6 * nc + i];
-                const double p = (gam - 1.0) * (q[7 * nc + i] - 0.5 * rho * (vx * vx + vy * vy + vz * vz) - 0.5 * BB2);
-                d->min_rho = std::min(d->min_rho, rho);
-                d->min_p = std::min(d->min_p, p);
-                if (rho == c->p.rhomin) d->n_rho_floor++;
-                if (std::fabs(p - c->p.pgmin) <= 1e-12 * std::fabs(c->p.pgmin)) d->n_p_floor++;
-                // CT face divergence: D4 of src/fluxct.cpp:848 along each axis
-                double dv = 0.0;
-                const int P0 = c->g.per[0], P1 = c->g.per[1], P2 = c->g.per[2];
-                if (nx > 1) {
-                    auto bx = [&](int xx) { return b[((size_t)z * ny + y) * nx + M(xx, nx, P0)]; };
-                    dv += (aa * (bx(x) - bx(x - 1)) + bb * (bx(x + 1) - bx(x - 2))) / c->p.dx;
-                }
-                if (ny > 1) {
-                    auto by = [&](int yy) { return b[nc + ((size_t)z * ny + M(yy, ny, P1)) * nx + x]; };
-                    dv += (aa * (by(y) - by(y - 1)) + bb * (by(y + 1) - by(y - 2))) / c->p.dy;
-                }
-                if (nz > 1) {
-                    auto bz = [&](int zz) { return b[2 * nc + ((size_t)M(zz, nz, P2) * ny + y) * nx + x]; };
-                    dv += (aa * (bz(z) - bz(z - 1)) + bb * (bz(z + 1) - bz(z - 2))) / c->p.dz;
-                }
-                d->max_divb_ct = std::max(d->max_divb_ct, std::fabs(dv));
-                // 2nd-order cell-centred diagnostic, src/output.cpp:314-347
-                double dc = 0.0;
-                if (nx > 1) {
-                    const int xm = M(x - 1, nx, P0), xp = M(x + 1, nx, P0);
-                    const double den = (xm == x || xp == x) ? c->p.dx : 2.0 * c->p.dx;
-                    dc += (q[4 * nc + ((size_t)z * ny + y) * nx + xp] - q[4 * nc + ((size_t)z * ny + y) * nx + xm]) / den;
-                }
-                if (ny > 1) {
-                    const int ym = M(y - 1, ny, P1), yp = M(y + 1, ny, P1);
-                    const double den = (ym == y || yp == y) ? c->p.dy : 2.0 * c->p.dy;
-                    dc += (q[5 * nc + ((size_t)z * ny + yp) * nx + x] - q[5 * nc + ((size_t)z * ny + ym) * nx + x]) / den;
-                }
-                if (nz > 1) {
-                    const int zm = M(z - 1, nz, P2), zp = M(z + 1, nz, P2);
-                    const double den = (zm == z || zp == z) ? c->p.dz : 2.0 * c->p.dz;
-                    dc += (q[6 * nc + ((size_t)zp * ny + y) * nx + x] - q[6 * nc + ((size_t)zm * ny + y) * nx + x]) / den;
-                }
-                d->max_divb_cell = std::max(d->max_divb_cell, std::fabs(dc));
-            }
+    for (int m = 0; m < 8; ++m) d->sum_cons[m] = h[DG_SUM + m];
+    d->max_divb_ct = h[DG_DIVCT];
+    d->max_divb_cell = h[DG_DIVC];
+    d->min_rho = -h[DG_NRHO];
+    d->min_p = -h[DG_NP];
+    d->n_rho_floor = (int64_t)h[DG_NFRHO];
+    d->n_p_floor = (int64_t)h[DG_NFP];
+    d->flagged_fraction = il[1] ? (double)il[0] / (double)il[1] : 0.0;
     return HOWMHD_OK;
 }
 
@@ -590,6 +574,8 @@ int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float thres
         HCHECK(c, hipMemcpy(c->mlp, &h, sizeof(h), hipMemcpyHostToDevice));
     }
     HCHECK(c, hipMemset(c->istats, 0, 4 * sizeof(unsigned long long)));
+    HCHECK(c, hipMemset(c->ilast, 0, 2 * sizeof(unsigned long long)));
+    c->any_routed = false;
     HCHECK(c, hipMemset(c->iwave, 0, (size_t)c->nwave * sizeof(uint4)));
     c->istats_pending = false;
     c->fallback_left = 0;
@@ -625,8 +611,6 @@ int howmhd_gpu_indicator_routing(howmhd_gpu_ctx* c, int64_t* routed_steps, int64
     return HOWMHD_OK;
 }
 
-const char* howmhd_gpu_build_kind(void) {
-    return "gfx950: fp64 reference rounding; fp64fast and fp32 with FMA contraction";
-}
+const char* howmhd_gpu_build_kind(void) { return kBuildKind; }
 
 }  // extern "C"

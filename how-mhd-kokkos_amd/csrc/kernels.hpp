@@ -1148,6 +1148,20 @@ __device__ __forceinline__ void block_max3(double m0, double m1, double m2, doub
         for (int d = 0; d < 3; ++d) out[d] = fmax(fmax(red[d][0], red[d][1]), fmax(red[d][2], red[d][3]));
 }
 
+// Counts of the lanes with a (b) set over the 256 threads of the block (all
+// must call it), in out[0..1].
+__device__ __forceinline__ void block_count2(bool a, bool b, unsigned* out) {
+    __shared__ unsigned cnt[2][4];
+    const int w = threadIdx.x / 64;
+    const unsigned ca = (unsigned)__popcll(__ballot(a)), cb = (unsigned)__popcll(__ballot(b));
+    if ((threadIdx.x & 63) == 0) { cnt[0][w] = ca; cnt[1][w] = cb; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = cnt[0][0] + cnt[0][1] + cnt[0][2] + cnt[0][3];
+        out[1] = cnt[1][0] + cnt[1][1] + cnt[1][2] + cnt[1][3];
+    }
+}
+
 template <typename R>
 struct CombineArgs {
     const double* q0;
@@ -1162,8 +1176,11 @@ struct CombineArgs {
     StageCoef s;
     Phys ph;
     // stage 5: per-block CFL maxima of the new q(0) (3 doubles per block),
-    // which the next tstep reduces instead of reading q(0) again; else null
+    // which the next tstep reduces instead of reading q(0) again, and the
+    // per-block counts of cells whose density and pressure floors prot
+    // applied (2 per block, read by the diagnostics); else null
     double* cfl;
+    unsigned* floors;
 };
 
 // 6th-order face -> centre interpolation of src/fluxct.cpp:891-896:
@@ -1181,6 +1198,7 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const Geom& g = A.g;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;  // CFL maxima (stage 5)
+    bool rho_floor = false, p_floor = false;  // prot raised rho / p (stage 5)
     if (i < g.ncell) {
         int x, y, z;
         cell_of(g, i, x, y, z);
@@ -1269,11 +1287,15 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
         }
         // prot
         const double gam = A.ph.gam;
+        // a floor is applied where the value is below it or not a number
+        // (fmax returns the floor for both)
+        rho_floor = !(v[0] >= A.ph.rhomin);
         double rho = fmax(v[0], A.ph.rhomin);
         const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
         const double vv2 = vx * vx + vy * vy + vz * vz;
         const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
         double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
+        p_floor = !(pg >= A.ph.pgmin);
         pg = fmax(pg, A.ph.pgmin);
         v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
         v[0] = rho;
@@ -1281,7 +1303,10 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
         for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
         if (A.cfl) cfl_cell(v, A.ph, m0, m1, m2);
     }
-    if (A.cfl) block_max3(m0, m1, m2, A.cfl + 3 * (long long)blockIdx.x);
+    if (A.cfl) {
+        block_max3(m0, m1, m2, A.cfl + 3 * (long long)blockIdx.x);
+        block_count2(rho_floor, p_floor, A.floors + 2 * (long long)blockIdx.x);
+    }
 }
 
 }  // namespace howmhd

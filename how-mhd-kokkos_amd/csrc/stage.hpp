@@ -190,6 +190,7 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     ca.s = s;
     ca.ph = c->ph;
     ca.cfl = (iw == 5) ? c->cflpart : nullptr;
+    ca.floors = (iw == 5) ? c->floorpart : nullptr;
     {
         TimedLaunch tl(c, HOWMHD_K_COMBINE);
         hipLaunchKernelGGL((k_combine<R, EX>), dim3(nb), dim3(256), 0, c->stream, ca);

@@ -49,9 +49,11 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 // Adds the per-wavefront counters of the routed sweeps (n slots of
 // (flagged, total, wavefronts flagged, wavefronts total)) to the four totals
-// and clears the slots.  Grid-stride loop over the slots; one atomic per
-// total and block.
-__global__ __launch_bounds__(256) void k_fold_wave_stats(uint4* w, long long n, unsigned long long* tot) {
+// and to (flagged, total) of the step in last[0..1] (cleared before the
+// launch), and clears the slots.  Grid-stride loop over the slots; one atomic
+// per total and block.
+__global__ __launch_bounds__(256) void k_fold_wave_stats(uint4* w, long long n, unsigned long long* tot,
+                                                         unsigned long long* last) {
     unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
         const uint4 v = w[i];
@@ -70,7 +72,152 @@ __global__ __launch_bounds__(256) void k_fold_wave_stats(uint4* w, long long n, 
     __syncthreads();
     if (threadIdx.x < 4) {
         const int k = threadIdx.x;
-        atomicAdd(&tot[k], red[k][0] + red[k][1] + red[k][2] + red[k][3]);
+        const unsigned long long v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        atomicAdd(&tot[k], v);
+        if (k < 2) atomicAdd(&last[k], v);
+    }
+}
+
+// ------------------------------------------------------------- diagnostics
+// Diagnostics of stage-0 q and faces (the divB field of src/output.cpp:154-188,
+// 314-347 and the CT face divergence), reduced on the device in FP64.  A
+// fixed grid of DIAG_BLOCKS blocks walks the cells; each block leaves its
+// partials (DIAG_NV values) and k_diag_fold combines them in block order,
+// so the sums do not depend on scheduling.
+constexpr int DIAG_BLOCKS = 1024;
+// sums of the 8 conserved variables, max |div_CT|, max |cell divB|,
+// -min rho, -min p, and the floor counts of the last stage-5 prot
+enum { DG_SUM = 0, DG_DIVCT = 8, DG_DIVC, DG_NRHO, DG_NP, DG_NFRHO, DG_NFP, DIAG_NV };
+
+struct DiagArgs {
+    const double* q;          // stage-0 q, 8 arrays of ncell
+    const double* b;          // stage-0 faces, 3 arrays of ncell
+    Geom g;
+    double gam, dx, dy, dz;
+    const unsigned* floors;   // 2 per block of the stage-5 combination, or null
+    long long nfloor;         // blocks of that combination
+    double* part;             // DIAG_BLOCKS x DIAG_NV
+};
+
+// Sum, or maximum, of v over the 256 threads of the block, in thread 0.
+template <bool MAX>
+__device__ __forceinline__ double block_reduce(double v, double* red4) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = MAX ? fmax(v, o) : v + o;
+    }
+    const int w = threadIdx.x / 64;
+    __syncthreads();  // red4 is free
+    if ((threadIdx.x & 63) == 0) red4[w] = v;
+    __syncthreads();
+    if (MAX) return fmax(fmax(red4[0], red4[1]), fmax(red4[2], red4[3]));
+    return (red4[0] + red4[1]) + (red4[2] + red4[3]);
+}
+
+__global__ __launch_bounds__(256) void k_diag(DiagArgs A) {
+    const Geom& g = A.g;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int px = g.per[0], py = g.per[1], pz = g.per[2];
+    const long long nc = g.ncell;
+    double sum[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double mct = 0.0, mcell = 0.0, nrho = -INFINITY, np = -INFINITY, frho = 0.0, fp = 0.0;
+    const double aa = 9.0 / 8.0, bb = -1.0 / 24.0;
+    const double* bx = A.b;
+    const double* by = A.b + nc;
+    const double* bz = A.b + 2 * nc;
+    const double* Bx = A.q + 4 * nc;
+    const double* By = A.q + 5 * nc;
+    const double* Bz = A.q + 6 * nc;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nc; i += (long long)gridDim.x * 256) {
+        int x, y, z;
+        cell_of(g, i, x, y, z);
+        double v[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            v[m] = A.q[m * nc + i];
+            sum[m] += v[m];
+        }
+        // pressure as src/output.cpp:300-312 forms it
+        const double vx = v[1] / v[0], vy = v[2] / v[0], vz = v[3] / v[0];
+        const double vv2 = vx * vx + vy * vy + vz * vz;
+        const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
+        const double pg = (A.gam - 1.0) * (v[7] - 0.5 * v[0] * vv2 - 0.5 * BB2);
+        nrho = fmax(nrho, -v[0]);
+        np = fmax(np, -pg);
+        // CT face divergence: D4 of src/fluxct.cpp:848 along each axis
+        double dv = 0.0;
+        if (nx > 1)
+            dv += (aa * (bx[i] - bx[lin(g, bmap(x - 1, nx, px), y, z)]) +
+                   bb * (bx[lin(g, bmap(x + 1, nx, px), y, z)] - bx[lin(g, bmap(x - 2, nx, px), y, z)])) / A.dx;
+        if (ny > 1)
+            dv += (aa * (by[i] - by[lin(g, x, bmap(y - 1, ny, py), z)]) +
+                   bb * (by[lin(g, x, bmap(y + 1, ny, py), z)] - by[lin(g, x, bmap(y - 2, ny, py), z)])) / A.dy;
+        if (nz > 1)
+            dv += (aa * (bz[i] - bz[lin(g, x, y, bmap(z - 1, nz, pz))]) +
+                   bb * (bz[lin(g, x, y, bmap(z + 1, nz, pz))] - bz[lin(g, x, y, bmap(z - 2, nz, pz))])) / A.dz;
+        mct = fmax(mct, fabs(dv));
+        // 2nd-order cell-centred divB, src/output.cpp:314-347: central
+        // difference, one-sided where the open edge clamps a neighbour
+        double dc = 0.0;
+        if (nx > 1) {
+            const int xm = bmap(x - 1, nx, px), xp = bmap(x + 1, nx, px);
+            const double den = (xm == x || xp == x) ? A.dx : 2.0 * A.dx;
+            dc += (Bx[lin(g, xp, y, z)] - Bx[lin(g, xm, y, z)]) / den;
+        }
+        if (ny > 1) {
+            const int ym = bmap(y - 1, ny, py), yp = bmap(y + 1, ny, py);
+            const double den = (ym == y || yp == y) ? A.dy : 2.0 * A.dy;
+            dc += (By[lin(g, x, yp, z)] - By[lin(g, x, ym, z)]) / den;
+        }
+        if (nz > 1) {
+            const int zm = bmap(z - 1, nz, pz), zp = bmap(z + 1, nz, pz);
+            const double den = (zm == z || zp == z) ? A.dz : 2.0 * A.dz;
+            dc += (Bz[lin(g, x, y, zp)] - Bz[lin(g, x, y, zm)]) / den;
+        }
+        mcell = fmax(mcell, fabs(dc));
+    }
+    if (A.floors)
+        for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < A.nfloor; k += (long long)gridDim.x * 256) {
+            frho += (double)A.floors[2 * k];
+            fp += (double)A.floors[2 * k + 1];
+        }
+    __shared__ double red4[4];
+    double* out = A.part + (long long)blockIdx.x * DIAG_NV;
+    double r;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        r = block_reduce<false>(sum[m], red4);
+        if (threadIdx.x == 0) out[DG_SUM + m] = r;
+    }
+    r = block_reduce<true>(mct, red4);
+    if (threadIdx.x == 0) out[DG_DIVCT] = r;
+    r = block_reduce<true>(mcell, red4);
+    if (threadIdx.x == 0) out[DG_DIVC] = r;
+    r = block_reduce<true>(nrho, red4);
+    if (threadIdx.x == 0) out[DG_NRHO] = r;
+    r = block_reduce<true>(np, red4);
+    if (threadIdx.x == 0) out[DG_NP] = r;
+    r = block_reduce<false>(frho, red4);
+    if (threadIdx.x == 0) out[DG_NFRHO] = r;
+    r = block_reduce<false>(fp, red4);
+    if (threadIdx.x == 0) out[DG_NFP] = r;
+}
+
+// Combines the nb block partials of k_diag into out[0..DIAG_NV-1] (one
+// block; thread t takes partials t, t + 256, ... in order).
+__global__ __launch_bounds__(256) void k_diag_fold(const double* part, int nb, double* out) {
+    __shared__ double red4[4];
+#pragma unroll 1
+    for (int v = 0; v < DIAG_NV; ++v) {
+        const bool mx = (v == DG_DIVCT || v == DG_DIVC || v == DG_NRHO || v == DG_NP);
+        double a = mx ? -INFINITY : 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) {
+            const double x = part[(long long)b * DIAG_NV + v];
+            a = mx ? fmax(a, x) : a + x;
+        }
+        const double r = mx ? block_reduce<true>(a, red4) : block_reduce<false>(a, red4);
+        if (threadIdx.x == 0) out[v] = r;
     }
 }
 

@@ -4,13 +4,16 @@
 // the parameters, initialise the problem on the host, hand q(0) to the GPU
 // library, build the face field, write output 0, then for each output
 // interval run CFL-limited hydro steps (dt clipped to the next output time,
-// abort on dt <= 0 or a non-finite time) and write the output.  Added: `nstep`
-// > 0 stops after that many hydro steps, a closing [timing] line reports
+// abort on dt <= 0 or a non-finite time) and write the output.  Added:
+// `max_steps` > 0 stops after that many hydro steps (`nstep` is parsed and
+// not used, as in the reference), a closing [timing] line reports
 // zone-cycles/s of the step loop, `checkpoint_every N` writes
 // checkpoint/ckptNNNNNN.bin (q(0), faces, output step, steps, time) after
 // every N-th output, and `restart <file>` resumes from such a file with the
-// next output interval.  A resumed run continues the uninterrupted one bit for
-// bit (the learned indicator's plain-WENO fallback starts afresh).
+// next output interval.  Without the learned indicator a resumed run
+// continues the uninterrupted one bit for bit; with it, the plain-WENO
+// fallback state is not in the checkpoint and starts afresh, so the resumed
+// run may route steps the uninterrupted run sent to the plain sweep.
 #include "howmhd_gpu.h"
 #include "howmhd_host.h"
 #include "howmhd_mlp.h"
@@ -170,7 +173,7 @@ int main(int argc, char* argv[]) {
         double dt = 0.0;
         const auto t0 = std::chrono::steady_clock::now();
         while (time < target_time) {
-            if (par.nstep > 0 && total_steps >= par.nstep) { stop = true; break; }
+            if (par.max_steps > 0 && total_steps >= par.max_steps) { stop = true; break; }
             check(howmhd_gpu_tstep(ctx, &dt), ctx, "tstep");
             if (time + dt > target_time) dt = target_time - time;
             if (dt <= 0.0) {

@@ -5,9 +5,9 @@
 // operator>>, a token starting with '#' comments out the rest of its line,
 // unknown keys warn on stderr and skip the rest of their line, and nx/ny/nz are
 // the global active-cell counts.  Keys this implementation adds: precision,
-// indicator, indicator_weights, seed, quiet, checkpoint_every, restart; nstep
-// gets the meaning "stop after nstep hydro steps" (the reference parses it and
-// never reads it).
+// indicator, indicator_weights, seed, quiet, checkpoint_every, restart,
+// max_steps ("stop after max_steps hydro steps").  nstep is parsed and, as in
+// the reference, not used.
 #include "howmhd_host.h"
 
 #include <cmath>
@@ -89,6 +89,7 @@ extern "C" int howmhd_host_parse(const char* text, howmhd_run* r, char* errbuf, 
         else if (key == "quiet") in >> r->quiet;
         else if (key == "checkpoint_every") in >> r->checkpoint_every;
         else if (key == "restart") { in >> s; set_str(r->restart, sizeof(r->restart), s); }
+        else if (key == "max_steps") in >> r->max_steps;
         else {
             std::cerr << "Warning: unknown parameter " << key << "\n";
             std::string dummy;

@@ -75,6 +75,7 @@ class Run(C.Structure):
         ("quiet", C.c_int32),
         ("checkpoint_every", C.c_int32),
         ("restart", C.c_char * 256),
+        ("max_steps", C.c_int32),
     ]
 
 

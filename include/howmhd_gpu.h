@@ -69,15 +69,21 @@ typedef struct howmhd_params {
     int32_t indicator;       /* HOWMHD_INDICATOR_*                                */
 } howmhd_params;
 
-/* Diagnostics of stage-0 state. */
+/* Diagnostics of stage-0 state, reduced on the device in FP64. */
 typedef struct howmhd_diag {
     double sum_cons[8];      /* sum over active cells of each conserved variable */
     double max_divb_ct;      /* max |sum_d D4_d(b_d)/Delta_d| over active cells   */
     double max_divb_cell;    /* max |2nd-order cell-centred divB| (output.cpp:314-347) */
-    double min_rho, min_p;
-    double flagged_fraction; /* indicator: flagged interfaces / all, last step   */
-    /* cells at the floors of prot (src/prot.cpp:31, 41): rho == rhomin, and
-     * p = (gam-1)(E - kinetic - magnetic) within 1e-12 (relative) of pgmin */
+    double min_rho;          /* min over active cells of rho                      */
+    double min_p;            /* min of (gam-1)(E - 0.5 rho v^2 - 0.5 B^2), formed as
+                                src/output.cpp:300-312 forms the pressure output  */
+    /* indicator: flagged interfaces / interfaces of the most recent step that
+     * routed (0 when no step has routed since set_indicator) */
+    double flagged_fraction;
+    /* cells whose density (rho < rhomin) and pressure (p < pgmin) floors the
+     * prot of the last step's final stage applied (src/prot.cpp:31, 41; a
+     * value that is not a number counts too, as fmax floors it); 0 before the
+     * first step after upload_q0 */
     int64_t n_rho_floor, n_p_floor;
 } howmhd_diag;
 
@@ -94,10 +100,13 @@ typedef struct howmhd_timing {
 
 typedef struct howmhd_gpu_ctx howmhd_gpu_ctx;
 
-/* Allocates q (5 stage buffers), faces (5 stage buffers) and the sweep / CT
- * scratch on the current HIP device.  Replaces the allocation of q
+/* Allocates q (5 stage buffers), faces (5 stage buffers), the sweep / CT
+ * scratch and a staging buffer for host transfers on the current HIP device;
+ * no later call allocates device memory.  Replaces the allocation of q
  * (src/main.cpp:114-120) and allocate_bfield_ct (src/fluxct.cpp:27-35).
- * Axes must have n == 1 (degenerate) or n >= 3. */
+ * Axes must have n == 1 (degenerate) or n >= 3; ny and nz at most 65535
+ * (launch grid limit), at most 2^40 cells.  Bad parameters return
+ * HOWMHD_EINVAL before any HIP call. */
 HOWMHD_API int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out);
 
 /* Frees everything.  Replaces deallocate_bfield_ct (src/fluxct.cpp:37-42). */
@@ -107,9 +116,9 @@ HOWMHD_API int howmhd_gpu_destroy(howmhd_gpu_ctx* ctx);
  * reference's problem initialisers write q(0,...) in place
  * (src/problem.cpp:39-59); this is the device hand-over of that array.
  * Host transfers (this call, upload_faces and the downloads) read or write
- * the caller's memory directly, one variable at a time, through a device
- * buffer of one variable's reference-layout size that the call allocates
- * and frees; the layout change runs on the device.  Synchronises. */
+ * the caller's memory directly, one variable at a time, through the
+ * context's device buffer of one variable's reference-layout size; the
+ * layout change runs on the device.  Synchronises. */
 HOWMHD_API int howmhd_gpu_upload_q0(howmhd_gpu_ctx* ctx, const double* q0);
 
 /* bound(q,0) + initialize_bfield_from_q + bound_bfield(0)
@@ -137,7 +146,11 @@ HOWMHD_API int howmhd_gpu_download_q0(howmhd_gpu_ctx* ctx, double* q0);
  * (src/fluxct.cpp:10-12 at stage 0).  Synchronises. */
 HOWMHD_API int howmhd_gpu_download_faces(howmhd_gpu_ctx* ctx, double* b);
 
-/* Diagnostics of the stage-0 state.  Synchronises. */
+/* Diagnostics of the stage-0 state: the cell-centred divB of the reference's
+ * output (src/output.cpp:154-188, 314-347), the CT face divergence, conserved
+ * sums, minima and floor counts.  Two kernels reduce them in FP64 with a
+ * fixed grid and a fixed combining order, so the same state gives the same
+ * bits; only the 13 results cross to the host.  Synchronises. */
 HOWMHD_API int howmhd_gpu_diag(howmhd_gpu_ctx* ctx, howmhd_diag* d);
 
 /* Starts (on != 0, counters reset) or stops per-kernel timing.  No
@@ -154,8 +167,9 @@ HOWMHD_API int howmhd_gpu_synchronize(howmhd_gpu_ctx* ctx);
  * when ctx is NULL).  Never NULL. */
 HOWMHD_API const char* howmhd_gpu_last_error(const howmhd_gpu_ctx* ctx);
 
-/* Build identification: "fp-exact" for the library built with FP contraction
- * off (bitwise twin of the CPU restatement), "fp-fast" for the product build. */
+/* Build identification: the target and the rounding of the precision forms
+ * ("gfx950: fp64 reference rounding; fp64fast and fp32 with FMA
+ * contraction"). */
 HOWMHD_API const char* howmhd_gpu_build_kind(void);
 
 #ifdef __cplusplus

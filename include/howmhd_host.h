@@ -20,7 +20,7 @@ extern "C" {
 typedef struct howmhd_run {
     int32_t gnx, gny, gnz;          /* nx|gnx, ny|gny, nz|gnz                  */
     int32_t nx, ny, nz;             /* local = global at one rank             */
-    int32_t nt, nstep, ntime;       /* nstep > 0: stop after nstep hydro steps */
+    int32_t nt, nstep, ntime;       /* parsed, unused (as in the reference)    */
     int32_t px, py, pz;
     double xsize, ysize, zsize;
     double dx, dy, dz;
@@ -36,6 +36,7 @@ typedef struct howmhd_run {
     int32_t quiet;                  /* 1: no per-step log lines                */
     int32_t checkpoint_every;       /* > 0: checkpoint after every such output */
     char restart[256];              /* checkpoint file to resume from          */
+    int32_t max_steps;              /* > 0: stop after max_steps hydro steps   */
 } howmhd_run;
 
 /* read_parameters + finalize_parameters_after_read

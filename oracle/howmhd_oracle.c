@@ -64,6 +64,9 @@ struct howmhd_ref_ctx {
     double* ct[12];        /* fsy fsz gsx gsz hsx hsy Ox2 Oy2 Oz2 Ox Oy Oz */
     double dt, dtdx, dtdy, dtdz;
     int route;             /* 0: characteristic WENO5 everywhere; 1: linear upwind-5 everywhere */
+    /* cells whose density / pressure floor the stage-5 prot of the last
+     * ssprk applied (test hook; the reference counts nothing) */
+    long long nfloor_rho, nfloor_p;
 };
 
 enum { FSY, FSZ, GSX, GSZ, HSX, HSY, OX2, OY2, OZ2, OX, OY, OZ };
@@ -876,10 +879,13 @@ static void fluxct(howmhd_ref_ctx* c, int iw, double k1, double k3) {
 }
 
 /* ================================================================== prot === */
-/* src/prot.cpp:6-54 */
+/* src/prot.cpp:6-54.  At stage 5 it also counts the cells where a floor
+ * takes effect: the value is below the floor or not a number (fmax returns
+ * the floor for both). */
 static void prot(howmhd_ref_ctx* c, int s) {
     const double gam = c->p.gam, rhomin = c->p.rhomin, pgmin = c->p.pgmin;
-#pragma omp parallel for collapse(2) schedule(static)
+    long long nr = 0, np = 0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : nr, np)
     for (int i = 3; i < c->p.nx + 3; ++i)
         for (int j = 3; j < c->p.ny + 3; ++j)
             for (int k = 3; k < c->p.nz + 3; ++k) {
@@ -887,16 +893,22 @@ static void prot(howmhd_ref_ctx* c, int s) {
                 const double Mx = Q(c, s, 1, i, j, k), My = Q(c, s, 2, i, j, k), Mz = Q(c, s, 3, i, j, k);
                 const double Bx = Q(c, s, 4, i, j, k), By = Q(c, s, 5, i, j, k), Bz = Q(c, s, 6, i, j, k);
                 double E = Q(c, s, 7, i, j, k);
+                nr += !(rho >= rhomin);
                 rho = fmax(rho, rhomin);
                 const double vx = Mx / rho, vy = My / rho, vz = Mz / rho;
                 const double vv2 = vx * vx + vy * vy + vz * vz;
                 const double BB2 = Bx * Bx + By * By + Bz * Bz;
                 double pg = (gam - 1.0) * (E - 0.5 * (rho * vv2 + BB2));
+                np += !(pg >= pgmin);
                 pg = fmax(pg, pgmin);
                 E = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
                 Q(c, s, 0, i, j, k) = rho;
                 Q(c, s, 7, i, j, k) = E;
             }
+    if (s == 5) {
+        c->nfloor_rho = nr;
+        c->nfloor_p = np;
+    }
 }
 
 /* ================================================================= ssprk === */
@@ -1136,6 +1148,13 @@ long long howmhd_ref_odd_sqrt_count(int reset) {
     const long long n = g_odd_sqrt;
     if (reset) g_odd_sqrt = 0;
     return n;
+}
+
+int howmhd_ref_floor_counts(const howmhd_ref_ctx* c, long long* n_rho, long long* n_p) {
+    if (!c || !n_rho || !n_p) return HOWMHD_EINVAL;
+    *n_rho = c->nfloor_rho;
+    *n_p = c->nfloor_p;
+    return 0;
 }
 
 int howmhd_ref_set_route(howmhd_ref_ctx* c, int route) {

@@ -39,6 +39,10 @@ int howmhd_ref_ssprk(howmhd_ref_ctx* c, double dt);
 int howmhd_ref_download_q0(howmhd_ref_ctx* c, double* q0);
 int howmhd_ref_download_faces(howmhd_ref_ctx* c, double* b);
 int howmhd_ref_set_threads(int n);
+/* Test hook: cells whose density and pressure floors the stage-5 prot of the
+ * last ssprk applied (value below the floor or not a number); 0 before the
+ * first ssprk. */
+int howmhd_ref_floor_counts(const howmhd_ref_ctx* c, long long* n_rho, long long* n_p);
 
 /* --- single-line and single-interface pieces, for the oracle's own checks --- */
 

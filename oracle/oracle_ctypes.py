@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
             "howmhd_ref_download_q0": (C.c_int, [C.c_void_p, _DP]),
             "howmhd_ref_download_faces": (C.c_int, [C.c_void_p, _DP]),
             "howmhd_ref_set_threads": (C.c_int, [C.c_int]),
+            "howmhd_ref_floor_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
             "howmhd_ref_line_sweep": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, _DP, _DP,
                                                 _DP, _DP]),
             "howmhd_ref_eigensystem": (None, [C.c_double, _DP, _DP, _DP, _DP]),
@@ -109,6 +110,12 @@ class RefSolver:
         out = np.empty((3,) + self.shape)
         self.lib.howmhd_ref_download_faces(self.h, _ptr(out))
         return out
+
+    def floor_counts(self) -> tuple[int, int]:
+        """(density, pressure) floors applied by the stage-5 prot of the last ssprk."""
+        nr, np_ = C.c_longlong(), C.c_longlong()
+        assert self.lib.howmhd_ref_floor_counts(self.h, C.byref(nr), C.byref(np_)) == 0
+        return nr.value, np_.value
 
     def set_route(self, route: int):
         """0: characteristic WENO5 everywhere; 1: linear upwind-5 everywhere."""

@@ -198,6 +198,10 @@ def test_routing_mixed_conserves(require_gpu):
     b1 = g.faces()
     scale = np.abs(H.active(b1)).max() / min(p.dx, p.dy, p.dz)
     assert np.abs(div(b1) - div(b0)).max() <= 1e-13 * scale
+    # the diagnostics report the flagged fraction of the most recent routed step
+    g.ssprk(g.tstep())
+    st1 = g.indicator_stats()
+    assert g.diag().flagged_fraction == st1["flagged"] / st1["total"]
 
 
 def test_routing_fp32_all_flagged_matches_plain_fp32(require_gpu):

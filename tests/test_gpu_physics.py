@@ -96,26 +96,110 @@ def test_conservation_and_ct_divergence_product_build(require_gpu):
     assert ddiv <= 1e-13 * scale
 
 
-def test_diag_floor_counts(require_gpu):
-    """howmhd_gpu_diag counts the cells that prot left at the density and
-    pressure floors; the counts equal those of the downloaded state."""
-    # pgmin at the ambient pressure: the rarefied cells sit on the floor
-    run, q0, p = _setup("blast3d", nx=16, ny=16, nz=8, rhomin=0.9999, pgmin=1.0)
+def _shift(a, axis, off, periodic):
+    """a at index i + off along axis, through the boundary map: wrap
+    (periodic) or clamp (open), as the ghost fills of src/bound.cpp leave it."""
+    n = a.shape[axis]
+    idx = np.arange(n) + off
+    idx = idx % n if periodic else np.clip(idx, 0, n - 1)
+    return np.take(a, idx, axis=axis)
+
+
+def _diag_from_state(q, b, p):
+    """The diagnostics of howmhd_gpu_diag computed in NumPy from a downloaded
+    state, each value formed as the reference forms it (src/output.cpp:154-188,
+    300-347; the D4 face divergence of src/fluxct.cpp:848)."""
+    a, f = H.active(q), H.active(b)
+    per = [p.bc[d] == H.BC["periodic"] for d in range(3)]
+    h = (p.dx, p.dy, p.dz)
+    n = (p.nx, p.ny, p.nz)
+    rho = a[0]
+    vx, vy, vz = a[1] / rho, a[2] / rho, a[3] / rho
+    vv2 = vx * vx + vy * vy + vz * vz
+    BB2 = a[4] * a[4] + a[5] * a[5] + a[6] * a[6]
+    pg = (p.gam - 1.0) * (a[7] - 0.5 * rho * vv2 - 0.5 * BB2)
+    aa, bb = 9.0 / 8.0, -1.0 / 24.0
+    dv = np.zeros_like(rho)
+    dc = np.zeros_like(rho)
+    for d in range(3):
+        if n[d] == 1:
+            continue
+        fd = f[d]
+        dv = dv + (aa * (fd - _shift(fd, d, -1, per[d])) + bb * (_shift(fd, d, 1, per[d]) - _shift(fd, d, -2, per[d]))) / h[d]
+        B = a[4 + d]
+        i = np.arange(n[d])
+        im = i - 1 if per[d] else np.clip(i - 1, 0, n[d] - 1)
+        ip = i + 1 if per[d] else np.clip(i + 1, 0, n[d] - 1)
+        one_sided = ((im % n[d]) == i) | ((ip % n[d]) == i)
+        den = np.where(one_sided, h[d], 2.0 * h[d])
+        shape = [1, 1, 1]
+        shape[d] = n[d]
+        dc = dc + (_shift(B, d, 1, per[d]) - _shift(B, d, -1, per[d])) / den.reshape(shape)
+    return {"sum": [math.fsum(a[m].ravel()) for m in range(8)],
+            "abs_sum": [float(np.abs(a[m]).sum()) for m in range(8)],
+            "max_divb_ct": float(np.abs(dv).max()), "max_divb_cell": float(np.abs(dc).max()),
+            "min_rho": float(rho.min()), "min_p": float(pg.min())}
+
+
+def _floor_blast():
+    """MHD blast 16^3 with the default floors (rhomin = pgmin = 1e-12), the
+    ambient pressure set to 1e-13 and a 3^3 pocket of density 1e-13: prot
+    applies both floors (the restatement counts 1 and 1137 cells at the first
+    step's final stage)."""
+    run, q0, p = _setup("mhd_blast3d", nx=16, ny=16, nz=16)
+    a = H.active(q0)
+    BB2 = a[4] ** 2 + a[5] ** 2 + a[6] ** 2
+    amb = (p.gam - 1.0) * (a[7] - 0.5 * BB2) < 1.0
+    a[7][amb] = 1e-13 / (p.gam - 1.0) + 0.5 * BB2[amb]
+    a[0][2:5, 2:5, 2:5] = 1e-13
+    return run, q0, p
+
+
+DIAG_CASES = {
+    "floors_default": (_floor_blast, 2),
+    "blast_open": (lambda: _setup("mhd_blast3d", nx=12, ny=14, nz=16, x1bc="open", x3bc="open"), 2),
+    "ot2d": (lambda: _setup("orszag_tang", nx=32, ny=32, nz=1, cour=0.8), 3),
+    "briowu_open": (lambda: _setup("brio_wu", nx=64, ny=1, nz=1, x1bc="open", x2bc="open", x3bc="open"), 4),
+}
+
+
+@pytest.mark.parametrize("case", list(DIAG_CASES))
+def test_diag_against_oracle_state(case, require_gpu):
+    """Every field of howmhd_gpu_diag (the FP64 device reduction) against
+    values computed from the restatement's state after each step (the plain
+    FP64 path equals it bit for bit): maxima, minima and the CT divergence
+    exactly, the sums to within the rounding of a reordered sum, and the floor
+    counts of the last stage-5 prot equal the restatement's counts, taken
+    before its fmax, with the default floors."""
+    make, nsteps = DIAG_CASES[case]
+    run, q0, p = make()
+    print(f"[diag] case={case} shape=({p.nx},{p.ny},{p.nz})", flush=True)
+    ref = O.RefSolver(p)
     g = H.GpuSolver(p)
-    g.upload_q0(q0)
-    g.init_faces()
-    for _ in range(2):
-        g.ssprk(g.tstep())
-    d = g.diag()
-    q = H.active(g.q0())
-    rho = q[0]
-    vx, vy, vz = q[1] / rho, q[2] / rho, q[3] / rho
-    pg = (p.gam - 1.0) * (q[7] - 0.5 * (rho * (vx * vx + vy * vy + vz * vz) + (q[4] ** 2 + q[5] ** 2 + q[6] ** 2)))
-    n_rho = int(np.count_nonzero(rho == p.rhomin))
-    n_p = int(np.count_nonzero(np.abs(pg - p.pgmin) <= 1e-12 * p.pgmin))
-    print("floor counts:", d.n_rho_floor, d.n_p_floor, "expected", n_rho, n_p)
-    assert n_rho > 0 and n_p > 0
-    assert (d.n_rho_floor, d.n_p_floor) == (n_rho, n_p)
+    for s in (ref, g):
+        s.upload_q0(q0)
+        s.init_faces()
+    seen_floors = [0, 0]
+    for step in range(nsteps + 1):
+        if step:
+            dt = ref.tstep()
+            ref.ssprk(dt)
+            g.ssprk(dt)
+        d = g.diag()
+        w = _diag_from_state(ref.q0(), ref.faces(), p)
+        for m in range(8):
+            assert abs(d.sum_cons[m] - w["sum"][m]) <= 1e-13 * w["abs_sum"][m], (step, m)
+        assert d.max_divb_ct == w["max_divb_ct"], (step, d.max_divb_ct, w["max_divb_ct"])
+        assert d.max_divb_cell == w["max_divb_cell"], (step, d.max_divb_cell, w["max_divb_cell"])
+        assert d.min_rho == w["min_rho"] and d.min_p == w["min_p"], step
+        want = ref.floor_counts() if step else (0, 0)
+        assert (d.n_rho_floor, d.n_p_floor) == want, (step, d.n_rho_floor, d.n_p_floor, want)
+        assert d.flagged_fraction == 0.0
+        seen_floors[0] += want[0]
+        seen_floors[1] += want[1]
+    print("floor counts over the steps (rho, p):", seen_floors, " last diag:", d.max_divb_ct, d.max_divb_cell)
+    if case == "floors_default":
+        assert seen_floors[0] > 0 and seen_floors[1] > 0
 
 
 def _alfven_error(n, solver_cls, **kw):
@@ -143,9 +227,10 @@ def test_alfven_wave_convergence_matches_restatement(require_gpu):
 
 def test_driver_matches_restatement(tmp_path, require_gpu):
     inp = tmp_path / "input.in"
-    inp.write_text("""# Orszag-Tang, two outputs
+    inp.write_text("""# Orszag-Tang, two outputs (nstep is parsed and not used, as in the reference)
 problem orszag_tang
 output_format vts
+nstep 1
 nx 32
 ny 32
 nz 1
@@ -187,6 +272,19 @@ nt 2
     assert np.abs(rho - want).sum() / np.abs(want).sum() <= 1e-12
     t_file = float(re.search(rb'Name="time" NumberOfTuples="1" format="ascii">\s*([0-9.e+-]+)', head).group(1))
     assert abs(t_file - time) <= 1e-14 * time
+
+
+def test_driver_max_steps(tmp_path, require_gpu):
+    """max_steps stops the run after that many hydro steps and writes the
+    output of the interval it stopped in at the time reached."""
+    inp = tmp_path / "input.in"
+    inp.write_text(H.make_input(problem="orszag_tang", nx=32, ny=32, nz=1, cour=0.8, tend=0.5, nt=1,
+                                output_format="none", max_steps=3))
+    out = subprocess.run([str(H.DRIVER), str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "hydro_steps = 3" in out.stdout
+    m = re.search(r"output step = 1 time = ([0-9.e+-]+)", out.stdout)
+    assert m and 0.0 < float(m.group(1)) < 0.5
 
 
 def test_fp32_turbulence_384_stays_close_to_fp64(require_gpu):

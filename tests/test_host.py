@@ -50,6 +50,17 @@ def test_create_fails_cleanly_without_gpu_or_with_bad_params():
         H.GpuSolver(p)
 
 
+@pytest.mark.parametrize("shape", [(4, 70000, 1), (4, 1, 65536), (1, 3, 70000)])
+def test_create_rejects_axes_beyond_the_launch_grid(shape):
+    """ny and nz go into gridDim.y / gridDim.z of the sweep, EMF and face
+    launches; create refuses lengths above 65535 before any HIP call (so this
+    runs without a GPU) instead of failing in the middle of a step."""
+    nx, ny, nz = shape
+    run = H.parse(H.make_input(problem="uniform", nx=nx, ny=ny, nz=nz, nt=1, tend=1))
+    with pytest.raises(H.HowmhdError, match="must not exceed 65535"):
+        H.GpuSolver(H.gpu_params(run))
+
+
 # ----------------------------------------------------------------- parser
 def test_parser_defaults_and_keys():
     text = """# comment line
@@ -74,6 +85,16 @@ output_format vts
     assert r.output_format == b"vts"
     assert r.dx == 1.0 / 16 and r.dy == 1.0 / 8 and r.dz == 1.0
     assert r.precision == b"fp64" and r.indicator == b"off"
+
+
+def test_parser_nstep_parsed_and_max_steps():
+    """nstep is read and otherwise unused, as in the reference
+    (src/parameters.cpp:58-59; nothing reads par.nstep); the early stop of the
+    driver has its own key, max_steps."""
+    r = H.parse("nx 4 ny 4 nz 4 nt 1 tend 1 nstep 5")
+    assert r.nstep == 5 and r.max_steps == 0
+    r = H.parse("nx 4 ny 4 nz 4 nt 1 tend 1 max_steps 7")
+    assert r.nstep == 0 and r.max_steps == 7
 
 
 def test_parser_gnx_alias_and_unknown_key(capfd):
