@@ -16,16 +16,24 @@
  * WENO5 + LLF against textbook Jiang-Shu WENO5; the eigensystem by R being
  * the eigenvectors of the flux Jacobian and L R = I; the sweeps by constant
  * states, axis relabelling and conservation; CT by the preserved D4
- * divergence; the whole step by convergence orders; tstep by its closed form
- * on a uniform state; the floors by their bounds.  The reference ships no
- * golden data; tests/golden holds regression fixtures of this file.
+ * divergence; the face initialisation by its 4th order, the face-to-centre
+ * interpolation by its 6th order and the corner EMFs by their 4th-order
+ * convergence to the cell-square average of the edge EMF (2nd order to its
+ * point value); the open fill of q by a one-stage identity with a periodic
+ * grid carrying the zero-gradient extension, and the open fill of faces and
+ * CT arrays by a two-step identity with a wider periodic grid where the flow
+ * is constant near the edges; the whole step by convergence orders; tstep by
+ * its closed form on a uniform state; the floors by their bounds.  The
+ * reference ships no golden data; tests/golden holds regression fixtures of
+ * this file.
  *
  * Parity unpinned (only the reference text fixes them; this file reproduces
  * them by reading it): the operation order inside each formula, the -0.0
  * weights of the CT flux terms, the stage-5 k4/k1 and k5/k1 divisions, prot
- * always rebuilding E, floors written back to the workspace only, and the
- * padded wrap of bc_index_ct (it only touches ghost results that are
- * refilled).
+ * always rebuilding E, floors written back to the workspace only, the padded
+ * wrap of bc_index_ct (it only touches ghost results that are refilled), and
+ * which face or CT value the open fill copies into the ghost layers where the
+ * flow is not constant near an open edge (bound_bfield and bound_ct_*).
  */
 #include "howmhd_oracle.h"
 
@@ -749,12 +757,14 @@ static int bci(int idx, int lo, int hi, int periodic) {
     return idx;
 }
 
-/* fluxct, src/fluxct.cpp:786-899 (serial boundary paths) */
-static void fluxct(howmhd_ref_ctx* c, int iw, double k1, double k3) {
+/* Corner EMFs from the six CT flux arrays, src/fluxct.cpp:795-842: ghost
+ * fill of the fluxes, 4th-order interpolation to the cell edges (O2), ghost
+ * fill, the transverse (delta^2)/24 correction (O), ghost fill.  Called with
+ * at least one in-plane pair of active axes. */
+static void ct_emf(howmhd_ref_ctx* c) {
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     const int use_x = nx > 1, use_y = ny > 1, use_z = nz > 1;
     const int use_Ox = use_x && use_y, use_Oy = use_y && use_z, use_Oz = use_x && use_z;
-    if (!use_Ox && !use_Oy && !use_Oz) return;
     const int xp = c->p.bc[0] == HOWMHD_BC_PERIODIC;
     const int yp = c->p.bc[1] == HOWMHD_BC_PERIODIC;
     const int zp = c->p.bc[2] == HOWMHD_BC_PERIODIC;
@@ -811,6 +821,21 @@ static void fluxct(howmhd_ref_ctx* c, int iw, double k1, double k3) {
                 }
             }
     for (int t = OX; t <= OZ; ++t) bound_a3(c, c->ct[t]);         /* bound_ct_emf */
+}
+
+/* fluxct, src/fluxct.cpp:786-899 (serial boundary paths) */
+static void fluxct(howmhd_ref_ctx* c, int iw, double k1, double k3) {
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const int use_x = nx > 1, use_y = ny > 1, use_z = nz > 1;
+    const int use_Ox = use_x && use_y, use_Oy = use_y && use_z, use_Oz = use_x && use_z;
+    if (!use_Ox && !use_Oy && !use_Oz) return;
+    const int xp = c->p.bc[0] == HOWMHD_BC_PERIODIC;
+    const int yp = c->p.bc[1] == HOWMHD_BC_PERIODIC;
+    const int zp = c->p.bc[2] == HOWMHD_BC_PERIODIC;
+    const int hx = nx + 5, hy = ny + 5, hz = nz + 5;
+    double* const* a = c->ct;
+
+    ct_emf(c);
 
     /* fluxct_update_bfield, 848-878 */
     const double aa = 9.0 / 8.0, bb = -1.0 / 24.0, cc = 0.0;
@@ -944,32 +969,35 @@ static void prepare_faces5(howmhd_ref_ctx* c, rkc k) {                          
     }
 }
 
+/* One stage of ssprk, src/ssprk.cpp:1281-1325 */
+static void stage(howmhd_ref_ctx* c, int iw) {
+    for (int t = 0; t < 12; ++t) memset(c->ct[t], 0, sizeof(double) * c->ncell);  /* zero_ct_fluxes */
+    const rkc k = rk_coeff(iw);
+    if (iw <= 4) {
+        copy_stage(c, iw, 0);
+        copy_faces(c, iw, 0);
+    } else {
+        prepare_stage5(c, k);
+        prepare_faces5(c, k);
+    }
+    bound_q(c, iw - 1);
+    if (iw == 5) bound_q(c, iw);
+    c->dtdx = k.k2 * c->dt / c->p.dx;
+    c->dtdy = k.k2 * c->dt / c->p.dy;
+    c->dtdz = k.k2 * c->dt / c->p.dz;
+    if (c->p.nx > 1) sweep(c, 0, iw, k, c->dtdx);
+    if (c->p.ny > 1) sweep(c, 1, iw, k, c->dtdy);
+    if (c->p.nz > 1) sweep(c, 2, iw, k, c->dtdz);
+    bound_faces(c, iw - 1);
+    fluxct(c, iw, k.k1, k.k3);
+    prot(c, iw);
+    bound_q(c, iw);
+    bound_faces(c, iw);
+}
+
 int howmhd_ref_ssprk(howmhd_ref_ctx* c, double dt) {
     c->dt = dt;
-    for (int iw = 1; iw <= 5; ++iw) {
-        for (int t = 0; t < 12; ++t) memset(c->ct[t], 0, sizeof(double) * c->ncell);  /* zero_ct_fluxes */
-        const rkc k = rk_coeff(iw);
-        if (iw <= 4) {
-            copy_stage(c, iw, 0);
-            copy_faces(c, iw, 0);
-        } else {
-            prepare_stage5(c, k);
-            prepare_faces5(c, k);
-        }
-        bound_q(c, iw - 1);
-        if (iw == 5) bound_q(c, iw);
-        c->dtdx = k.k2 * c->dt / c->p.dx;
-        c->dtdy = k.k2 * c->dt / c->p.dy;
-        c->dtdz = k.k2 * c->dt / c->p.dz;
-        if (c->p.nx > 1) sweep(c, 0, iw, k, c->dtdx);
-        if (c->p.ny > 1) sweep(c, 1, iw, k, c->dtdy);
-        if (c->p.nz > 1) sweep(c, 2, iw, k, c->dtdz);
-        bound_faces(c, iw - 1);
-        fluxct(c, iw, k.k1, k.k3);
-        prot(c, iw);
-        bound_q(c, iw);
-        bound_faces(c, iw);
-    }
+    for (int iw = 1; iw <= 5; ++iw) stage(c, iw);
     copy_stage(c, 0, 5);
     copy_faces(c, 0, 5);
     return 0;
@@ -1154,6 +1182,28 @@ int howmhd_ref_floor_counts(const howmhd_ref_ctx* c, long long* n_rho, long long
     if (!c || !n_rho || !n_p) return HOWMHD_EINVAL;
     *n_rho = c->nfloor_rho;
     *n_p = c->nfloor_p;
+    return 0;
+}
+
+int howmhd_ref_ct_emf(howmhd_ref_ctx* c, const double* ct6, double* o3) {
+    if (!c || !ct6 || !o3) return HOWMHD_EINVAL;
+    const int use_x = c->p.nx > 1, use_y = c->p.ny > 1, use_z = c->p.nz > 1;
+    if (!((use_x && use_y) || (use_y && use_z) || (use_x && use_z))) return HOWMHD_EINVAL;
+    for (int t = 0; t < 12; ++t) memset(c->ct[t], 0, sizeof(double) * c->ncell);
+    for (int t = FSY; t <= HSY; ++t) memcpy(c->ct[t], ct6 + (size_t)(t - FSY) * c->ncell, sizeof(double) * c->ncell);
+    ct_emf(c);
+    for (int t = OX; t <= OZ; ++t) memcpy(o3 + (size_t)(t - OX) * c->ncell, c->ct[t], sizeof(double) * c->ncell);
+    return 0;
+}
+
+int howmhd_ref_ssprk_stages(howmhd_ref_ctx* c, double dt, int nst, double* q_out, double* b_out) {
+    if (!c || !q_out || !b_out || nst < 1 || nst > 5) return HOWMHD_EINVAL;
+    c->dt = dt;
+    for (int iw = 1; iw <= nst; ++iw) stage(c, iw);
+    memcpy(q_out, &c->q[(size_t)nst * NVAR * c->ncell], sizeof(double) * NVAR * c->ncell);
+    memcpy(b_out, c->bx + (size_t)nst * c->ncell, sizeof(double) * c->ncell);
+    memcpy(b_out + c->ncell, c->by + (size_t)nst * c->ncell, sizeof(double) * c->ncell);
+    memcpy(b_out + 2 * c->ncell, c->bz + (size_t)nst * c->ncell, sizeof(double) * c->ncell);
     return 0;
 }
 

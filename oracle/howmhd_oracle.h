@@ -44,6 +44,18 @@ int howmhd_ref_set_threads(int n);
  * first ssprk. */
 int howmhd_ref_floor_counts(const howmhd_ref_ctx* c, long long* n_rho, long long* n_p);
 
+/* Test hook: the corner-EMF construction of one CT update on given flux
+ * arrays (src/fluxct.cpp:795-842, the code fluxct runs).  ct6 holds the
+ * padded arrays fsy, fsz, gsx, gsz, hsx, hsy (6 x (nx+6)(ny+6)(nz+6)); o3
+ * receives the smoothed corner EMFs Ox, Oy, Oz after their ghost fill.  A
+ * context with fewer than two active axes returns HOWMHD_EINVAL. */
+int howmhd_ref_ct_emf(howmhd_ref_ctx* c, const double* ct6, double* o3);
+
+/* Test hook: the first nst (1..5) stages of one ssprk step with the given dt;
+ * q_out (8 x cells) and b_out (3 x cells) receive stage nst's padded state.
+ * q(0) and the faces of stage 0 are left as they are. */
+int howmhd_ref_ssprk_stages(howmhd_ref_ctx* c, double dt, int nst, double* q_out, double* b_out);
+
 /* --- single-line and single-interface pieces, for the oracle's own checks --- */
 
 /* One directional sweep of one line (src/ssprk.cpp:458-997): qline holds

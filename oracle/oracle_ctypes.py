@@ -41,6 +41,8 @@ def lib() -> C.CDLL:
             "howmhd_ref_download_faces": (C.c_int, [C.c_void_p, _DP]),
             "howmhd_ref_set_threads": (C.c_int, [C.c_int]),
             "howmhd_ref_floor_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+            "howmhd_ref_ct_emf": (C.c_int, [C.c_void_p, _DP, _DP]),
+            "howmhd_ref_ssprk_stages": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _DP, _DP]),
             "howmhd_ref_line_sweep": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, _DP, _DP,
                                                 _DP, _DP]),
             "howmhd_ref_eigensystem": (None, [C.c_double, _DP, _DP, _DP, _DP]),
@@ -116,6 +118,27 @@ class RefSolver:
         nr, np_ = C.c_longlong(), C.c_longlong()
         assert self.lib.howmhd_ref_floor_counts(self.h, C.byref(nr), C.byref(np_)) == 0
         return nr.value, np_.value
+
+    def ct_emf(self, ct6):
+        """Smoothed corner EMFs (Ox, Oy, Oz) that the CT update builds from the
+        six padded flux arrays ct6 = (fsy, fsz, gsx, gsz, hsx, hsy)."""
+        ct6 = np.ascontiguousarray(ct6, dtype=np.float64)
+        assert ct6.shape == (6,) + self.shape
+        out = np.empty((3,) + self.shape)
+        rc = self.lib.howmhd_ref_ct_emf(self.h, _ptr(ct6), _ptr(out))
+        if rc != 0:
+            raise RuntimeError(f"howmhd_ref_ct_emf failed ({rc})")
+        return out
+
+    def ssprk_stages(self, dt: float, nst: int):
+        """(q, faces) of stage nst after the first nst stages of a step; q(0)
+        and the stage-0 faces are unchanged."""
+        q = np.empty((8,) + self.shape)
+        b = np.empty((3,) + self.shape)
+        rc = self.lib.howmhd_ref_ssprk_stages(self.h, dt, nst, _ptr(q), _ptr(b))
+        if rc != 0:
+            raise RuntimeError(f"howmhd_ref_ssprk_stages failed ({rc})")
+        return q, b
 
     def set_route(self, route: int):
         """0: characteristic WENO5 everywhere; 1: linear upwind-5 everywhere."""

@@ -10,7 +10,12 @@ sign, a swapped index or permutation) fails:
   sweeps             constant data -> zero increments; axis permutation of a
                      3-D state -> permuted result; conservation of mass,
                      momentum and energy on periodic grids
-  CT                 the D4 face divergence is preserved to round-off
+  CT                 the D4 face divergence is preserved to round-off; face
+                     init 4th order, face-to-centre 6th order, corner EMFs
+                     4th order to the cell-square average of the edge EMF
+  open fill          one-stage identity with the zero-gradient extension
+                     (q); two-step identity with a wider periodic twin where
+                     the flow is constant near the edges (q and faces)
   whole step         uniform state stays uniform; 4th/5th-order convergence on
                      a smooth circularly polarised Alfven wave
   tstep              closed form on a uniform state
@@ -391,3 +396,299 @@ def test_restatement_regression_fixtures():
         assert got["dt"] == case["dt"], case["problem"]
         assert got["q_sha256"] == case["q_sha256"], case["problem"]
         assert got["faces_sha256"] == case["faces_sha256"], case["problem"]
+
+
+# ------------------------------------- CT interpolations and the open fill
+# Orders of accuracy that the mathematics fixes for the interpolation weights
+# of the CT update, and the domain-of-dependence identity of the open
+# (zero-gradient) ghost fill.  A wrong weight, a dropped term, a sign slip or
+# an interpolation along the wrong axis lowers an order to 2 or less, or
+# breaks the bitwise identity.
+
+def _params(n, bc=(0, 0, 0), L=(1.0, 1.0, 1.0), gam=GAM):
+    p = H.Params()
+    p.nx, p.ny, p.nz = n
+    for d in range(3):
+        p.bc[d] = bc[d]
+    p.dx, p.dy, p.dz = (L[d] / n[d] for d in range(3))
+    p.gam, p.cour, p.rhomin, p.pgmin = gam, 0.4, 1e-12, 1e-12
+    return p
+
+
+def _coords(p, stagger):
+    """Coordinates of every padded index (c - 3 + 0.5) h, shifted by h/2 along
+    the axes in `stagger` (face or edge positions: index c is at c + 1/2)."""
+    n = (p.nx, p.ny, p.nz)
+    h = (p.dx, p.dy, p.dz)
+    ax = [(np.arange(n[d] + 6) - 2.5 + (0.5 if d in stagger else 0.0)) * h[d] for d in range(3)]
+    return np.meshgrid(*ax, indexing="ij")
+
+
+# smooth periodic test functions on the unit cube, each with its own wave
+# vector so that an interpolation along the wrong axis is visible
+_KV = [np.array(k) * 2 * np.pi for k in ((1, 2, 3), (3, 1, 2), (2, 3, 1), (1, 3, 2), (2, 1, 3), (3, 2, 1))]
+_PH = (0.0, 0.3, 0.7, 1.1, 1.9, 2.3)
+
+
+def _wave(t, X, Y, Z):
+    k = _KV[t]
+    return np.sin(k[0] * X + k[1] * Y + k[2] * Z + _PH[t])
+
+
+def _order(errs):
+    return [math.log2(errs[i] / errs[i + 1]) for i in range(len(errs) - 1)]
+
+
+def test_face_init_is_fourth_order():
+    """initialize_bfield_from_q (src/fluxct.cpp:76-102): face values from the
+    cell-centred field with (-1, 9, 9, -1)/16 along the face normal.  On smooth
+    data they converge to the field at the face at 4th order."""
+    errs = {0: [], 1: [], 2: []}
+    for n in (16, 32, 64):
+        p = _params((n, n, n))
+        Xc = _coords(p, ())
+        q0 = np.zeros((8,) + Xc[0].shape)
+        B = [0.1 * _wave(d, *Xc) for d in range(3)]
+        q0[0] = 1.0
+        q0[4:7] = B
+        q0[7] = 1.0 / (GAM - 1) + 0.5 * (B[0] ** 2 + B[1] ** 2 + B[2] ** 2)
+        s = O.RefSolver(p)
+        s.upload_q0(q0)
+        s.init_faces()
+        b = s.faces()
+        s.close()
+        for d in range(3):
+            want = 0.1 * _wave(d, *_coords(p, (d,)))
+            errs[d].append(np.abs(H.active(b)[d] - want[3:-3, 3:-3, 3:-3]).max())
+    for d in range(3):
+        orders = _order(errs[d])
+        print("face init, component", d, "max errors", errs[d], "orders", orders)
+        assert all(3.8 < o < 4.2 for o in orders), (d, errs[d], orders)
+
+
+def test_face_to_centre_is_sixth_order():
+    """fluxct_interp_b_to_q (src/fluxct.cpp:882-897): cell B from the faces with
+    (3, -25, 150, 150, -25, 3)/256.  A step with dt = 0 leaves the faces as
+    they are (the stage weights k1 + k3 + k4 + k5 sum to 1), and writes cell B
+    from them at every stage; on smooth face data cell B converges to the
+    field at the cell centre at 6th order."""
+    errs = {0: [], 1: [], 2: []}
+    for n in (8, 16, 32):
+        p = _params((n, n, n))
+        Xc = _coords(p, ())
+        faces = np.stack([0.1 * _wave(d, *_coords(p, (d,))) for d in range(3)])
+        Bc = [0.1 * _wave(d, *Xc) for d in range(3)]
+        q0 = np.zeros((8,) + Xc[0].shape)
+        q0[0] = 1.0
+        q0[4:7] = Bc
+        q0[7] = 1.0 / (GAM - 1) + 0.5 * (Bc[0] ** 2 + Bc[1] ** 2 + Bc[2] ** 2)
+        s = O.RefSolver(p)
+        s.upload_q0(q0)
+        s.upload_faces(faces)
+        s.ssprk(0.0)
+        q = s.q0()
+        b = s.faces()
+        s.close()
+        # the faces stay where they were, up to the rounding of the stage sums
+        assert np.abs(H.active(b) - faces[:, 3:-3, 3:-3, 3:-3]).max() <= 1e-15
+        for d in range(3):
+            errs[d].append(np.abs(H.active(q)[4 + d] - Bc[d][3:-3, 3:-3, 3:-3]).max())
+    for d in range(3):
+        orders = _order(errs[d])
+        print("face to centre, component", d, "max errors", errs[d], "orders", orders)
+        assert all(5.7 < o < 6.3 for o in orders), (d, errs[d], orders)
+
+
+def _square_average(t, X, Y, Z, axes, h):
+    """Average of test function t over the h x h square of the two axes in
+    `axes` centred at (X, Y, Z): the point value times sinc(k h / 2) per axis."""
+    f = _wave(t, X, Y, Z)
+    for d in axes:
+        u = _KV[t][d] * h[d] / 2
+        f = f * (np.sin(u) / u)
+    return f
+
+
+def test_corner_emf_is_fourth_order_transverse_average():
+    """Corner EMFs of the CT update (src/fluxct.cpp:809-840): O2 interpolates the
+    two flux arrays of each edge to the edge with (-1, 9, 9, -1)/16, and O adds
+    (delta^2 O2)/24 along the two in-plane axes.  On smooth flux data O
+    converges at 4th order to the average of (first - second flux) over the
+    cell-sized square around the edge, and only at 2nd order to its point
+    value (the correction is what turns point values into square averages).
+
+    Flux arrays and positions (index c sits at c + 1/2 along the sweep axis):
+    x sweep fsy, fsz at x faces; y sweep gsx, gsz at y faces; z sweep hsx, hsy
+    at z faces.  Ox = I4_x(gsx) - I4_y(fsy) at z edges (x+1/2, y+1/2),
+    Oy = I4_y(hsy) - I4_z(gsz) at x edges (y+1/2, z+1/2), Oz = I4_z(fsz) -
+    I4_x(hsx) at y edges (x+1/2, z+1/2)."""
+    # (array, staggered axis) in the hook's order fsy fsz gsx gsz hsx hsy
+    stag = (0, 0, 1, 1, 2, 2)
+    # (first, second, edge axes) per output Ox, Oy, Oz
+    edges = ((2, 0, (0, 1)), (5, 3, (1, 2)), (1, 4, (0, 2)))
+    errs_avg = {0: [], 1: [], 2: []}
+    errs_pt = {0: [], 1: [], 2: []}
+    for n in (24, 48, 96):
+        p = _params((n, n, n))
+        h = (p.dx, p.dy, p.dz)
+        ct6 = np.stack([_wave(t, *_coords(p, (stag[t],))) for t in range(6)])
+        s = O.RefSolver(p)
+        o3 = s.ct_emf(ct6)
+        s.close()
+        for e, (f1, f2, axes) in enumerate(edges):
+            Xe = _coords(p, axes)
+            want = _square_average(f1, *Xe, axes, h) - _square_average(f2, *Xe, axes, h)
+            point = _wave(f1, *Xe) - _wave(f2, *Xe)
+            got = H.active(o3)[e]
+            errs_avg[e].append(np.abs(got - want[3:-3, 3:-3, 3:-3]).max())
+            errs_pt[e].append(np.abs(got - point[3:-3, 3:-3, 3:-3]).max())
+    for e in range(3):
+        oa, op = _order(errs_avg[e]), _order(errs_pt[e])
+        print("EMF", "xyz"[e], "vs square average", errs_avg[e], oa, "vs point value", errs_pt[e], op)
+        assert all(3.8 < o < 4.2 for o in oa), (e, errs_avg[e], oa)
+        assert all(1.8 < o < 2.2 for o in op), (e, errs_pt[e], op)
+
+
+def _two_state_flow(p, axis, lo, hi, width):
+    """States A (cells below `lo` along `axis`) and B (cells above `hi`),
+    joined by a smooth ramp; the flow moves along `axis` (v_axis = 0.6) and
+    the ramp region also varies along the other two axes, so that all sweeps
+    and the CT update act there.  Cell index i counts active cells from 0."""
+    n = (p.nx, p.ny, p.nz)
+    idx = np.meshgrid(*[np.arange(n[d] + 6) - 3.0 for d in range(3)], indexing="ij")
+    s = idx[axis]
+    w = np.clip((s - lo) / (hi - lo), 0.0, 1.0)
+    w = w * w * (3 - 2 * w)
+    o1, o2 = [d for d in range(3) if d != axis]
+    bump = np.where((s > lo - width) & (s < hi + width),
+                    0.05 * np.sin(2 * np.pi * idx[o1] / n[o1]) * np.cos(2 * np.pi * idx[o2] / n[o2]), 0.0)
+    rho = 1.0 + 0.5 * w + bump
+    pg = 1.0 + 0.2 * w
+    v = np.zeros((3,) + rho.shape)
+    v[axis] = 0.6
+    v[o1] = 0.1 - 0.05 * w
+    B = np.zeros((3,) + rho.shape)
+    B[axis] = 0.5
+    B[o1] = 0.3 + 0.1 * w
+    B[o2] = 0.2
+    q0 = np.zeros((8,) + rho.shape)
+    q0[0] = rho
+    q0[1:4] = rho * v
+    q0[4:7] = B
+    q0[7] = pg / (GAM - 1) + 0.5 * rho * (v * v).sum(0) + 0.5 * (B * B).sum(0)
+    return q0
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_open_fill_equals_wider_periodic_twin(axis):
+    """Open boundaries (src/bound.cpp:34-47, 64-77, 94-107; faces and CT arrays
+    src/fluxct.cpp:141-205, 510-552) extend the edge cell into the ghost
+    layers.  Where the flow is constant within the domain of dependence of an
+    open edge, that is exactly what a wider periodic domain holds there, so
+    the open run and the matching window of the wider periodic twin agree bit
+    for bit, q and faces, after two steps with the same dt.  A reflecting
+    fill (normal momentum or field negated), a zero fill or a periodic wrap
+    (A on one edge, B on the other) breaks the identity.  The ramp region
+    lies 150 cells from both open edges and the window 160 cells from the
+    twin's wrap; two steps change the flow only within 8 cells of the ramp
+    (bitwise, measured), far below both distances."""
+    N, M, lo, hi, width = 320, 160, 156, 164, 6
+    shape = [6, 5, 4]
+    shape[axis] = N
+    bc = [0, 0, 0]
+    bc[axis] = 1
+    p_open = _params(tuple(shape), tuple(bc), L=tuple(float(n) for n in shape))
+    tshape = list(shape)
+    tshape[axis] = N + 2 * M
+    p_twin = _params(tuple(tshape), (0, 0, 0), L=tuple(float(n) for n in tshape))
+    q_open = _two_state_flow(p_open, axis, lo, hi, width)
+    q_twin = _two_state_flow(p_twin, axis, lo + M, hi + M, width)
+    so, st = O.RefSolver(p_open), O.RefSolver(p_twin)
+    so.upload_q0(q_open)
+    st.upload_q0(q_twin)
+    so.init_faces()
+    st.init_faces()
+    for _ in range(2):
+        dt = so.tstep()
+        so.ssprk(dt)
+        st.ssprk(dt)
+    sl = [slice(3, -3)] * 3
+    sl[axis] = slice(3 + M, 3 + M + N)
+    qo, qt = H.active(so.q0()), st.q0()[(slice(None),) + tuple(sl)]
+    bo, bt = H.active(so.faces()), st.faces()[(slice(None),) + tuple(sl)]
+    # the ramp has moved and the window holds both states, so the check is not empty
+    assert np.abs(qo[0] - q_open[0][3:-3, 3:-3, 3:-3]).max() > 1e-3
+    assert qo[0].min() < 1.01 and qo[0].max() > 1.49
+    np.testing.assert_array_equal(qo, qt)
+    np.testing.assert_array_equal(bo, bt)
+    so.close()
+    st.close()
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_open_fill_is_constant_extension_one_stage(axis):
+    """The open fill of q (src/bound.cpp:34-47, 64-77, 94-107) is the
+    zero-gradient extension: each of the three ghost layers holds the edge
+    cell.  On a 1-D flow that varies right up to both edges, the first stage
+    of a step on the open grid therefore equals, bit for bit, the first stage
+    on a periodic grid that carries the edge values 8 cells further out on
+    each side (the jump where that grid wraps around is more than one stage's
+    stencil from the window).  A fill from the second or third cell in (a
+    mirror), a reflection or a periodic wrap breaks the identity.  (In 1-D the
+    CT update is off, so this pins the fill of q; the fill of faces and CT
+    arrays is pinned where the flow is constant near the edge, by
+    test_open_fill_equals_wider_periodic_twin.)"""
+    N, M = 24, 8
+    shape = [1, 1, 1]
+    shape[axis] = N
+    bc = [0, 0, 0]
+    bc[axis] = 1
+    p_open = _params(tuple(shape), tuple(bc), L=tuple(float(n) for n in shape))
+    tshape = list(shape)
+    tshape[axis] = N + 2 * M
+    p_twin = _params(tuple(tshape), (0, 0, 0), L=tuple(float(n) for n in tshape))
+    x = (np.arange(N) + 0.5) / N
+    o1, o2 = [d for d in range(3) if d != axis]
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * x + 0.4)
+    pg = 1.0 + 0.2 * np.cos(2 * np.pi * x)
+    v = np.zeros((3, N))
+    v[axis] = 0.6 + 0.1 * np.sin(4 * np.pi * x)
+    v[o1] = 0.2 * np.cos(2 * np.pi * x + 1.0)
+    v[o2] = -0.1 * np.sin(2 * np.pi * x)
+    B = np.zeros((3, N))
+    B[axis] = 0.5
+    B[o1] = 0.3 + 0.1 * np.sin(2 * np.pi * x + 0.2)
+    B[o2] = 0.2 * np.cos(2 * np.pi * x + 0.5)
+    line = np.zeros((8, N))
+    line[0] = rho
+    line[1:4] = rho * v
+    line[4:7] = B
+    line[7] = pg / (GAM - 1) + 0.5 * rho * (v * v).sum(0) + 0.5 * (B * B).sum(0)
+    wide = np.concatenate([np.repeat(line[:, :1], M, 1), line, np.repeat(line[:, -1:], M, 1)], axis=1)
+
+    def padded(lin, n):
+        shp = [7, 7, 7]
+        shp[axis] = n + 6
+        q = np.zeros((8,) + tuple(shp))
+        sl = [3, 3, 3]
+        sl[axis] = slice(3, n + 3)
+        q[(slice(None),) + tuple(sl)] = lin
+        return q
+
+    so, st = O.RefSolver(p_open), O.RefSolver(p_twin)
+    so.upload_q0(padded(line, N))
+    st.upload_q0(padded(wide, N + 2 * M))
+    so.init_faces()
+    st.init_faces()
+    dt = so.tstep()
+    qo, _ = so.ssprk_stages(dt, 1)
+    qt, _ = st.ssprk_stages(dt, 1)
+    sl = [3, 3, 3]
+    sl[axis] = slice(3, N + 3)
+    got = qo[(slice(None),) + tuple(sl)]
+    sl[axis] = slice(3 + M, 3 + M + N)
+    want = qt[(slice(None),) + tuple(sl)]
+    assert np.abs(got - line).max() > 1e-4          # the stage moved the flow
+    np.testing.assert_array_equal(got, want)
+    so.close()
+    st.close()
