@@ -21,6 +21,7 @@ struct howmhd_gpu_ctx {
     howmhd_params p{};
     howmhd::Geom g{};
     howmhd::Phys ph{};
+    howmhd::GasConst gc{};
     howmhd::CtFlags ctf{};
     size_t ncell = 0;
     size_t rsize = 8;          // bytes of the sweep/CT real type

@@ -201,6 +201,10 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     c->g.ncell = (long long)c->ncell;
     c->g.sx = 1; c->g.sy = n[0]; c->g.sz = n[0] * n[1];
     c->ph.gam = p->gam; c->ph.rhomin = p->rhomin; c->ph.pgmin = p->pgmin;
+    // IEEE quotients on the host: the bits the device's divisions give
+    c->gc.gam2 = (p->gam - 2.0) / (p->gam - 1.0);
+    c->gc.gm1 = p->gam - 1.0;
+    c->gc.inv_gm1 = 1.0 / c->gc.gm1;
     c->ctf.use_x = n[0] > 1; c->ctf.use_y = n[1] > 1; c->ctf.use_z = n[2] > 1;
     c->ctf.use_Ox = c->ctf.use_x && c->ctf.use_y;
     c->ctf.use_Oy = c->ctf.use_y && c->ctf.use_z;
@@ -218,9 +222,12 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(HOWMHD_EHIP, "hipStreamCreate failed");
     // a failed allocation is handled here: the runtime's last error is
-    // cleared so that it does not surface at a later launch check
+    // cleared so that it does not surface at a later launch check.  The
+    // clears run on the context's stream: the stream does not wait for the
+    // null stream (hipStreamNonBlocking), and a clear there could still be
+    // running when the first upload writes the buffer.
     auto alloc = [&](void** ptr, size_t bytes) -> bool {
-        if (hipMalloc(ptr, bytes) != hipSuccess || hipMemset(*ptr, 0, bytes) != hipSuccess) {
+        if (hipMalloc(ptr, bytes) != hipSuccess || hipMemsetAsync(*ptr, 0, bytes, c->stream) != hipSuccess) {
             (void)hipGetLastError();
             return false;
         }
@@ -267,6 +274,8 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
                    hipSuccess;
     ok = ok && hipEventCreateWithFlags(&c->istats_ev, hipEventDisableTiming) == hipSuccess;
     if (!ok) return cleanup_fail(HOWMHD_ENOMEM, "howmhd_gpu_create: device allocation failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+        return cleanup_fail(HOWMHD_EHIP, "howmhd_gpu_create: clearing the device buffers failed");
     *out = c;
     return HOWMHD_OK;
 }
@@ -571,12 +580,16 @@ int howmhd_gpu_set_indicator(howmhd_gpu_ctx* c, const float* params, float thres
             }
         h.b3 = params[MLP_OFF_B3];
         h.logit_threshold = std::log(threshold / (1.f - threshold));
-        HCHECK(c, hipMemcpy(c->mlp, &h, sizeof(h), hipMemcpyHostToDevice));
+        // on the context's stream (which does not wait for the null stream),
+        // complete before h goes out of scope
+        HCHECK(c, hipMemcpyAsync(c->mlp, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+        HCHECK(c, hipStreamSynchronize(c->stream));
     }
-    HCHECK(c, hipMemset(c->istats, 0, 4 * sizeof(unsigned long long)));
-    HCHECK(c, hipMemset(c->ilast, 0, 2 * sizeof(unsigned long long)));
+    HCHECK(c, hipMemsetAsync(c->istats, 0, 4 * sizeof(unsigned long long), c->stream));
+    HCHECK(c, hipMemsetAsync(c->ilast, 0, 2 * sizeof(unsigned long long), c->stream));
     c->any_routed = false;
-    HCHECK(c, hipMemset(c->iwave, 0, (size_t)c->nwave * sizeof(uint4)));
+    HCHECK(c, hipMemsetAsync(c->iwave, 0, (size_t)c->nwave * sizeof(uint4), c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
     c->istats_pending = false;
     c->fallback_left = 0;
     for (int k = 0; k < 4; ++k) c->acc[k] = 0;
@@ -592,8 +605,9 @@ int howmhd_gpu_indicator_stats(howmhd_gpu_ctx* c, int64_t* flagged, int64_t* tot
     const int rc = fold_indicator_stats(c, true);
     if (rc) return rc;
     unsigned long long h[4];
-    HCHECK(c, hipMemcpy(h, c->istats, sizeof(h), hipMemcpyDeviceToHost));
-    HCHECK(c, hipMemset(c->istats, 0, sizeof(h)));
+    HCHECK(c, hipMemcpyAsync(h, c->istats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HCHECK(c, hipMemsetAsync(c->istats, 0, sizeof(h), c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
     long long s[4] = {c->acc[0], c->acc[1], c->acc[2], c->acc[3]};
     for (int k = 0; k < 4; ++k) s[k] += (long long)h[k];
     for (int k = 0; k < 4; ++k) c->acc[k] = 0;

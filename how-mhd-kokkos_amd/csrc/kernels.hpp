@@ -50,6 +50,17 @@ __device__ __forceinline__ int bmap(int a, int n, int per) {
     return a < 0 ? 0 : (a >= n ? n - 1 : a);
 }
 
+// Coordinates of stored index i, and the stored index of (x, y, z).
+__device__ __forceinline__ void cell_of(const Geom& g, long long i, int& x, int& y, int& z) {
+    x = (int)(i % g.n[0]);
+    const long long r = i / g.n[0];
+    y = (int)(r % g.n[1]);
+    z = (int)(r / g.n[1]);
+}
+__device__ __forceinline__ long long lin(const Geom& g, int x, int y, int z) {
+    return ((long long)z * g.n[1] + y) * g.n[0] + x;
+}
+
 // Sweep permutation of src/ssprk.cpp:381-403, 1035-1065, 1126-1156: sweep slot m
 // holds q variable PERM[dir][m]; order (rho, Mn, Mt1, Mt2, Bn, Bt1, Bt2, E).
 __device__ __constant__ const int kPerm[3][8] = {
@@ -120,6 +131,7 @@ struct SweepArgs {
     R* ct_b;            // CT term "bsz": fsz (x), gsx (y), hsy (z)
     Geom g;
     Phys ph;
+    GasConst gc;
     // SW_COLLECT: 18 features and the label per interface (sample_index), and
     // (when logit is set) the in-sweep network's logit
     float* feat;
@@ -220,7 +232,7 @@ struct Seven {
 template <typename R, int DIR, int MODE, int DM>
 __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>::CELLS],
                                                const R (*lo)[SweepTile<DIR>::CELLS], int row, int lane, R gam,
-                                               R (&dFo)[7], R& mdev, unsigned long long& bad) {
+                                               const GasConst& gc, R (&dFo)[7], R& mdev, unsigned long long& bad) {
     using T = SweepTile<DIR>;
     (void)mdev;
     auto slot = [&](int k) -> int {
@@ -237,9 +249,11 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
         u1[4] = lo[O_BN][s1]; u1[5] = luf[4][s1].u; u1[6] = luf[5][s1].u; u1[7] = lo[O_P][s1]; u1[8] = lo[O_H][s1];
     }
     Eig<R> e;
-    eig_scalars<R>(gam, u0, u1, e);
-    // the unscaled mode normalises the L rows with div_pos: needs 1/cs2 > 0
-    if constexpr (DM == DIV_UNSCALED) bad |= __ballot(!(e.nrm_half > R(0)));
+    bool sqbad = false;
+    eig_scalars<R, DM>(gam, u0, u1, e, &sqbad);
+    // the unscaled mode normalises the L rows with div_pos (needs 1/cs2 > 0)
+    // and takes unscaled square roots (operands 0 or >= 2^-767)
+    if constexpr (DM == DIV_UNSCALED) bad |= __ballot(!(e.nrm_half > R(0)) || sqbad);
 
     // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
     Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
@@ -271,8 +285,8 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
         // rows and columns here instead of all seven up front (98 live values)
         opaque(e);
         R La[7], Lb[7];
-        l_row<m0, R, DM>(gam, e, La);
-        if constexpr (two) l_row<m1, R, DM>(gam, e, Lb);
+        l_row<m0, R, DM>(gam, e, La, gc);
+        if constexpr (two) l_row<m1, R, DM>(gam, e, Lb, gc);
         R Fa[6], qa[6], Fb[6], qb[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -303,11 +317,11 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
         opaque_after(e, Fsa);
         if constexpr (two) opaque_after(e, Fsb);
         R Rc[7];
-        r_col<m0>(gam, e, Rc);
+        r_col<m0>(gam, e, Rc, gc);
 #pragma unroll
         for (int r = 0; r < 7; ++r) dFo[r] += Rc[r] * Fsa;
         if constexpr (two) {
-            r_col<m1>(gam, e, Rc);
+            r_col<m1>(gam, e, Rc, gc);
 #pragma unroll
             for (int r = 0; r < 7; ++r) dFo[r] += Rc[r] * Fsb;
         }
@@ -476,7 +490,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
     R maxdev = R(0);
     if (MODE != SW_ROUTED || flagged) {
         unsigned long long bad = 0ull;
-        weno_interface<R, DIR, MODE, DM>(luf, lo, row, lane, R(A.ph.gam), dF, maxdev, bad);
+        weno_interface<R, DIR, MODE, DM>(luf, lo, row, lane, R(A.ph.gam), A.gc, dF, maxdev, bad);
         // a WENO weight operand beyond the unscaled-division range: the tile
         // goes (once) on the list of the IEEE-division launch that follows
         if constexpr (EX && DM == DIV_UNSCALED) {
@@ -583,15 +597,6 @@ struct EmfArgs {
     CtFlags f;
 };
 
-__device__ __forceinline__ void cell_of(const Geom& g, long long i, int& x, int& y, int& z) {
-    x = (int)(i % g.n[0]);
-    const long long r = i / g.n[0];
-    y = (int)(r % g.n[1]);
-    z = (int)(r / g.n[1]);
-}
-__device__ __forceinline__ long long lin(const Geom& g, int x, int y, int z) {
-    return ((long long)z * g.n[1] + y) * g.n[0] + x;
-}
 
 // 4-point interpolation of src/fluxct.cpp:815-820: (-a + 9b + 9c - d)/16.
 template <typename R>
@@ -1189,9 +1194,114 @@ __device__ __forceinline__ double i6(double a, double b, double c, double d, dou
     return div_c<double>(3.0 * a - 25.0 * b + 150.0 * c + 150.0 * d - 25.0 * e + 3.0 * f, 256.0, 1.0 / 256.0);
 }
 
-// Stage combination: x_sweep's store q = k1*q + k3*q_prev + dX
-// (src/ssprk.cpp:415-430), then += dY, += dZ (1075-1094, 1166-1185); face ->
-// centre B (fluxct.cpp:882-897); prot (prot.cpp:10-51).
+// Stage combination of one cell (x, y, z; linear index i): x_sweep's store
+// q = k1*q + k3*q_prev + dX (src/ssprk.cpp:415-430), then += dY, += dZ
+// (1075-1094, 1166-1185); face -> centre B (fluxct.cpp:882-897); prot
+// (prot.cpp:10-51).  v receives the new q of the cell; rho_floor / p_floor
+// whether prot raised rho / p (a value below the floor or not a number:
+// fmax returns the floor for both).
+template <typename R>
+__device__ __forceinline__ void combine_cell(const CombineArgs<R>& A, long long i, int x, int y, int z, double* v,
+                                             bool& rho_floor, bool& p_floor) {
+    const Geom& g = A.g;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const StageCoef& s = A.s;
+    // With CT active the cell-centred B (m = 4..6) is replaced by the face
+    // interpolation below, so its flux update is not formed (its dF
+    // components are not stored either).
+    const bool ct = A.f.active;
+    auto upd = [&](int m) { return !(ct && m >= 4 && m <= 6); };
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const long long o = (long long)m * g.ncell + i;
+        v[m] = 0.0;
+        if (upd(m)) v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
+    }
+    // slot of each q variable in the dF of each sweep (-1: the normal field)
+    // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
+    // y: rho->0, Mx->3, My->1, Mz->2, Bx->5, By->-, Bz->4, E->6
+    // z: rho->0, Mx->2, My->3, Mz->1, Bx->4, By->5, Bz->-, E->6
+    constexpr int sx[8] = {0, 1, 2, 3, -1, 4, 5, 6};
+    constexpr int sy[8] = {0, 3, 1, 2, 5, -1, 4, 6};
+    constexpr int sz[8] = {0, 2, 3, 1, 4, 5, -1, 6};
+    if (nx > 1) {
+        const int tl = (x == 0 && g.per[0]) ? nx : x, tr = x + 1;
+        const double dt = s.dtdx;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            if (!upd(m)) continue;
+            double d = 0.0;
+            if (sx[m] >= 0)
+                d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
+            v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
+        }
+    }
+    if (ny > 1) {
+        const int tl = (y == 0 && g.per[1]) ? ny : y, tr = y + 1;
+        const double dt = s.dtdy;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            if (!upd(m)) continue;
+            double d = 0.0;
+            if (sy[m] >= 0)
+                d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] - (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
+            v[m] = v[m] + d;
+        }
+    }
+    if (nz > 1) {
+        const int tl = (z == 0 && g.per[2]) ? nz : z, tr = z + 1;
+        const double dt = s.dtdz;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            if (!upd(m)) continue;
+            double d = 0.0;
+            if (sz[m] >= 0)
+                d = -dt * ((double)A.dFz[dF_index<2>(g, sz[m], tr, x, y, z)] - (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, z)]);
+            v[m] = v[m] + d;
+        }
+    }
+    if (A.f.active) {
+        const double* bx = A.bo;
+        const double* by = A.bo + g.ncell;
+        const double* bz = A.bo + 2 * g.ncell;
+        if (A.f.use_x) {
+            const int p = g.per[0];
+            v[4] = i6(bx[lin(g, bmap(x - 3, nx, p), y, z)], bx[lin(g, bmap(x - 2, nx, p), y, z)], bx[lin(g, bmap(x - 1, nx, p), y, z)],
+                      bx[i], bx[lin(g, bmap(x + 1, nx, p), y, z)], bx[lin(g, bmap(x + 2, nx, p), y, z)]);
+        } else {
+            v[4] = bx[i];
+        }
+        if (A.f.use_y) {
+            const int p = g.per[1];
+            v[5] = i6(by[lin(g, x, bmap(y - 3, ny, p), z)], by[lin(g, x, bmap(y - 2, ny, p), z)], by[lin(g, x, bmap(y - 1, ny, p), z)],
+                      by[i], by[lin(g, x, bmap(y + 1, ny, p), z)], by[lin(g, x, bmap(y + 2, ny, p), z)]);
+        } else {
+            v[5] = by[i];
+        }
+        if (A.f.use_z) {
+            const int p = g.per[2];
+            v[6] = i6(bz[lin(g, x, y, bmap(z - 3, nz, p))], bz[lin(g, x, y, bmap(z - 2, nz, p))], bz[lin(g, x, y, bmap(z - 1, nz, p))],
+                      bz[i], bz[lin(g, x, y, bmap(z + 1, nz, p))], bz[lin(g, x, y, bmap(z + 2, nz, p))]);
+        } else {
+            v[6] = bz[i];
+        }
+    }
+    // prot
+    const double gam = A.ph.gam;
+    rho_floor = !(v[0] >= A.ph.rhomin);
+    double rho = fmax(v[0], A.ph.rhomin);
+    const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
+    const double vv2 = vx * vx + vy * vy + vz * vz;
+    const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
+    double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
+    p_floor = !(pg >= A.ph.pgmin);
+    pg = fmax(pg, A.ph.pgmin);
+    v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
+    v[0] = rho;
+}
+
+// The stage combination over every cell (one thread per cell); at stage 5
+// also the per-block CFL maxima and floor counts.
 template <typename R, int EX>
 __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     static_assert(EX == HOWMHD_EXACT, "kernel instantiated in a unit of the other rounding form");
@@ -1202,103 +1312,8 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
     if (i < g.ncell) {
         int x, y, z;
         cell_of(g, i, x, y, z);
-        const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-        const StageCoef& s = A.s;
-        // With CT active the cell-centred B (m = 4..6) is replaced by the face
-        // interpolation below, so its flux update is not formed (its dF
-        // components are not stored either).
-        const bool ct = A.f.active;
-        auto upd = [&](int m) { return !(ct && m >= 4 && m <= 6); };
         double v[8];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const long long o = (long long)m * g.ncell + i;
-            v[m] = 0.0;
-            if (upd(m)) v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
-        }
-        // slot of each q variable in the dF of each sweep (-1: the normal field)
-        // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
-        // y: rho->0, Mx->3, My->1, Mz->2, Bx->5, By->-, Bz->4, E->6
-        // z: rho->0, Mx->2, My->3, Mz->1, Bx->4, By->5, Bz->-, E->6
-        constexpr int sx[8] = {0, 1, 2, 3, -1, 4, 5, 6};
-        constexpr int sy[8] = {0, 3, 1, 2, 5, -1, 4, 6};
-        constexpr int sz[8] = {0, 2, 3, 1, 4, 5, -1, 6};
-        if (nx > 1) {
-            const int tl = (x == 0 && g.per[0]) ? nx : x, tr = x + 1;
-            const double dt = s.dtdx;
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                if (!upd(m)) continue;
-                double d = 0.0;
-                if (sx[m] >= 0)
-                    d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
-                v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
-            }
-        }
-        if (ny > 1) {
-            const int tl = (y == 0 && g.per[1]) ? ny : y, tr = y + 1;
-            const double dt = s.dtdy;
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                if (!upd(m)) continue;
-                double d = 0.0;
-                if (sy[m] >= 0)
-                    d = -dt * ((double)A.dFy[dF_index<1>(g, sy[m], tr, x, y, z)] - (double)A.dFy[dF_index<1>(g, sy[m], tl, x, y, z)]);
-                v[m] = v[m] + d;
-            }
-        }
-        if (nz > 1) {
-            const int tl = (z == 0 && g.per[2]) ? nz : z, tr = z + 1;
-            const double dt = s.dtdz;
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                if (!upd(m)) continue;
-                double d = 0.0;
-                if (sz[m] >= 0)
-                    d = -dt * ((double)A.dFz[dF_index<2>(g, sz[m], tr, x, y, z)] - (double)A.dFz[dF_index<2>(g, sz[m], tl, x, y, z)]);
-                v[m] = v[m] + d;
-            }
-        }
-        if (A.f.active) {
-            const double* bx = A.bo;
-            const double* by = A.bo + g.ncell;
-            const double* bz = A.bo + 2 * g.ncell;
-            if (A.f.use_x) {
-                const int p = g.per[0];
-                v[4] = i6(bx[lin(g, bmap(x - 3, nx, p), y, z)], bx[lin(g, bmap(x - 2, nx, p), y, z)], bx[lin(g, bmap(x - 1, nx, p), y, z)],
-                          bx[i], bx[lin(g, bmap(x + 1, nx, p), y, z)], bx[lin(g, bmap(x + 2, nx, p), y, z)]);
-            } else {
-                v[4] = bx[i];
-            }
-            if (A.f.use_y) {
-                const int p = g.per[1];
-                v[5] = i6(by[lin(g, x, bmap(y - 3, ny, p), z)], by[lin(g, x, bmap(y - 2, ny, p), z)], by[lin(g, x, bmap(y - 1, ny, p), z)],
-                          by[i], by[lin(g, x, bmap(y + 1, ny, p), z)], by[lin(g, x, bmap(y + 2, ny, p), z)]);
-            } else {
-                v[5] = by[i];
-            }
-            if (A.f.use_z) {
-                const int p = g.per[2];
-                v[6] = i6(bz[lin(g, x, y, bmap(z - 3, nz, p))], bz[lin(g, x, y, bmap(z - 2, nz, p))], bz[lin(g, x, y, bmap(z - 1, nz, p))],
-                          bz[i], bz[lin(g, x, y, bmap(z + 1, nz, p))], bz[lin(g, x, y, bmap(z + 2, nz, p))]);
-            } else {
-                v[6] = bz[i];
-            }
-        }
-        // prot
-        const double gam = A.ph.gam;
-        // a floor is applied where the value is below it or not a number
-        // (fmax returns the floor for both)
-        rho_floor = !(v[0] >= A.ph.rhomin);
-        double rho = fmax(v[0], A.ph.rhomin);
-        const double vx = v[1] / rho, vy = v[2] / rho, vz = v[3] / rho;
-        const double vv2 = vx * vx + vy * vy + vz * vz;
-        const double BB2 = v[4] * v[4] + v[5] * v[5] + v[6] * v[6];
-        double pg = (gam - 1.0) * (v[7] - 0.5 * (rho * vv2 + BB2));
-        p_floor = !(pg >= A.ph.pgmin);
-        pg = fmax(pg, A.ph.pgmin);
-        v[7] = pg / (gam - 1.0) + 0.5 * (rho * vv2 + BB2);
-        v[0] = rho;
+        combine_cell<R>(A, i, x, y, z, v, rho_floor, p_floor);
 #pragma unroll
         for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
         if (A.cfl) cfl_cell(v, A.ph, m0, m1, m2);

@@ -124,6 +124,38 @@ __device__ __forceinline__ R div_q(R a, R b, R y) {
     else return div_by<R>(a, b, y);
 }
 
+// FP64 square root without its range scaling, for the reference-rounding
+// sweep proper.  gfx9's correctly rounded square root scales an operand below
+// 2^-767 by 2^256 (and the result by 2^-128), refines v_rsq_f64 with the
+// steps below, and returns the operand itself for +-0 and +inf.  For every
+// other operand >= 2^-767, and for NaN, the scaling is the identity, so these
+// steps give its bits; an operand in (0, 2^-767) or negative raises `bad`,
+// and the caller recomputes the tile with sqrt().
+__device__ __forceinline__ double sqrt_unscaled(double x, bool& bad) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r;
+    double h = r * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    const double d0 = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, e, h);
+    g = __builtin_fma(d0, h, g);
+    const double d1 = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d1, h, g);
+    const bool special = __builtin_amdgcn_class(x, 0x260);  // -0, +0, +inf
+    bad = bad || (x < 0x1p-767 && !special);
+    return special ? x : g;
+}
+
+// Square root of the reference-rounding sweep in division mode DM: the
+// unscaled form (raising `bad` out of its range) in the sweep proper, else
+// the IEEE square root.
+template <int DM>
+__device__ __forceinline__ double sqrt_q(double x, bool& bad) {
+    if constexpr (DM == DIV_UNSCALED) return sqrt_unscaled(x, bad);
+    else return sqrt(x);
+}
+
 // x / c for a compile-time constant c > 0 with inv_c = RN(1/c).  Reference
 // rounding: the correctly rounded quotient of div_by without its select:
 // with nr = fma(c, q, -x) = -r exactly, fma(-nr, inv_c, q) equals div_by's
@@ -161,8 +193,9 @@ enum CellField {
 // Input: conserved in sweep order, FP64.  The primitive conversion, the
 // floors and E -> p stay in FP64; the outputs are rounded to R.
 // DM: division mode of the reference-rounding form; in the unscaled mode
-// *bad is raised for a cell whose shared reciprocals are not positive (it is
-// then recomputed with div_by).
+// *bad is raised for a cell whose shared reciprocals are not positive or
+// whose square-root operands leave the range of sqrt_unscaled (it is then
+// recomputed with div_by and sqrt()).
 template <typename R, int DM = DIV_IEEE>
 __device__ __forceinline__ void cell_values(const double* qs, double gam, double rhomin, double pgmin,
                                             R* out, bool* bad = nullptr) {
@@ -218,10 +251,15 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
     const R bbn2 = div_q<DM>(rBB2, rrho, irho);
     const R bnx2 = div_q<DM>(rBx * rBx, rrho, irho);
     const R cs2 = rmax(R(0), rgam * rabs(div_q<DM>(rpg, rrho, irho)));
-    const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
-    const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
+    // root and lf take the unscaled square root; la and ls keep the IEEE one:
+    // their operands scale with B_n^2, which a blast front's precursor takes
+    // below 2^-767 on whole tiles (they would all be recomputed)
+    bool sq_bad = false;
+    const R root = sqrt_q<DM>(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2), sq_bad);
+    const R lf = sqrt_q<DM>(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)), sq_bad);
     const R la = rsqrt_(rmax(R(0), bnx2));
     const R ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
+    if constexpr (DM == DIV_UNSCALED) *bad = *bad || sq_bad;
 #else
     R lf, la, ls;
     if constexpr (sizeof(R) == 8) {
@@ -265,6 +303,15 @@ __device__ __forceinline__ void cell_values(const double* qs, double gam, double
 }
 
 // ------------------------------------------------------ interface eigensystem
+// Constants of the equation of state the eigenvectors use, computed once on
+// the host in FP64 with the reference's expressions: gam2 = (gam-2)/(gam-1)
+// (src/ssprk.cpp:729-787), gm1 = gam-1 and inv_gm1 = RN(1/gm1).  The IEEE
+// quotient is the same on the host and the device, so gam2 has the bits the
+// reference computes per interface.
+struct GasConst {
+    double gam2, gm1, inv_gm1;
+};
+
 // Scalars of the interface eigensystem (src/ssprk.cpp:610-659).  Rows of L
 // and columns of R are built from them one wave at a time.
 template <typename R>
@@ -304,8 +351,16 @@ __device__ __forceinline__ void eig_scalars_fp32_speeds(Eig<R>& e, R cs2, R bn2,
     }
 }
 
-template <typename R>
-__device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig<R>& e) {
+// DM: division mode of the reference-rounding form; in the unscaled mode the
+// square roots take sqrt_unscaled and raise *sqbad out of its range.
+template <typename R, int DM = DIV_IEEE>
+__device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig<R>& e, bool* sqbad = nullptr) {
+#if HOWMHD_EXACT
+    bool sb = false;
+    auto sq = [&](R x) -> R { return sqrt_q<DM>(x, sb); };
+#else
+    auto sq = [](R x) -> R { return rsqrt_(x); };
+#endif
     // u = (rho, vn, vt1, vt2, Bn, Bt1, Bt2, p, H) of the two cells
     const R rho = R(0.5) * (u0[0] + u1[0]);
     const R vx = R(0.5) * (u0[1] + u1[1]);
@@ -326,7 +381,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     const R bnx2 = Bx * Bx * irho;
 #endif
     const R cs2 = (gam - R(1)) * (HH - R(0.5) * (vv2 + bbn2));
-    const R cs = rsqrt_(rmax(R(0), cs2));
+    const R cs = sq(rmax(R(0), cs2));
     const R Bt2 = By * By + Bz * Bz;
 #if !HOWMHD_EXACT
     if constexpr (sizeof(R) == 4) {
@@ -334,15 +389,20 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     } else
 #endif
     {
-    const R root = rsqrt_(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
-    const R lf = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
+    // unscaled square roots (sq) only where the operand stays far above
+    // 2^-767 in MHD flows: c_s^2, root^2, c_f^2, c_f^2 - c_slow^2 (>= 1e-30
+    // here), B_t^2 (>= 1e-30) and rho.  c_a^2 and c_slow^2 scale with B_n^2,
+    // and the alpha numerators vanish where B_t does: IEEE square roots.
+    const R root = sq(rmax(R(0), (bbn2 + cs2) * (bbn2 + cs2) - R(4) * bnx2 * cs2));
+    const R lf = sq(rmax(R(0), R(0.5) * (bbn2 + cs2 + root)));
     const R ls = rsqrt_(rmax(R(0), R(0.5) * (bbn2 + cs2 - root)));
     e.lf = lf; e.ls = ls;
     R af, as;
     if ((lf * lf - ls * ls) >= R(1.0e-30)) {
 #if HOWMHD_EXACT
-        af = rsqrt_(rmax(R(0), cs2 - ls * ls)) / rsqrt_(lf * lf - ls * ls);
-        as = rsqrt_(rmax(R(0), lf * lf - cs2)) / rsqrt_(lf * lf - ls * ls);
+        const R sd = sq(lf * lf - ls * ls);
+        af = rsqrt_(rmax(R(0), cs2 - ls * ls)) / sd;
+        as = rsqrt_(rmax(R(0), lf * lf - cs2)) / sd;
 #else
         const R rd = rcp_fast(rsqrt_(lf * lf - ls * ls));
         af = rsqrt_(rmax(R(0), cs2 - ls * ls)) * rd;
@@ -358,8 +418,9 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     R bty, btz;
     if (Bt2 >= R(1.0e-30)) {
 #if HOWMHD_EXACT
-        bty = By / rsqrt_(Bt2);
-        btz = Bz / rsqrt_(Bt2);
+        const R sbt = sq(Bt2);
+        bty = By / sbt;
+        btz = Bz / sbt;
 #else
         const R rb = rcp_fast(rsqrt_(Bt2));
         bty = By * rb;
@@ -369,7 +430,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
         bty = R(1) / rsqrt_(R(2));
         btz = R(1) / rsqrt_(R(2));
     }
-    const R sqrtrho = rsqrt_(rho);
+    const R sqrtrho = sq(rho);
     e.vx = vx; e.vy = vy; e.vz = vz; e.By = By; e.Bz = Bz; e.vv2 = vv2;
     e.cs = cs; e.cs2 = cs2;
     e.bty = bty; e.btz = btz; e.sgnBx = sign1(Bx);
@@ -388,6 +449,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
 #else
     e.nrm_half = R(1) / cs2;   // RN(1/cs2): the L-row normalisation divides by cs2 through div_by
     e.nrm_ent = R(0);
+    if (sqbad) *sqbad = *sqbad || sb;
 #endif
 }
 
@@ -396,7 +458,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
 // DM: division mode of the reference-rounding normalisation (div_q; the
 // unscaled mode needs e.nrm_half > 0, which the caller checks).
 template <int M, typename R, int DM = DIV_IEEE>
-__device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
+__device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L, const GasConst& gc) {
     const R gam0 = R(1) - gam;
     const R gam1 = R(0.5) * (gam - R(1));
     const R af = e.af, as = e.as, lf = e.lf, ls = e.ls, cs = e.cs, cs2 = e.cs2;
@@ -428,7 +490,11 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
         L[6] = -gam0 * as;
     } else if constexpr (M == 3) {
 #if HOWMHD_EXACT
-        L[0] = -cs2 / gam0 - R(0.5) * vv2;
+        // -cs2/gam0 = cs2/(gam-1) exactly (RN(1-gam) = -RN(gam-1)); the
+        // sweep proper divides by the constant through its reciprocal
+        // (div_pos: correctly rounded, barring a subnormal cs2)
+        if constexpr (DM == DIV_UNSCALED) L[0] = div_pos<R>(cs2, R(gc.gm1), R(gc.inv_gm1)) - R(0.5) * vv2;
+        else L[0] = -cs2 / gam0 - R(0.5) * vv2;
 #else
         L[0] = -cs2 * rcp_fast(gam0) - R(0.5) * vv2;
 #endif
@@ -492,10 +558,12 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L) {
 
 // Column m of R after the sign flip (src/ssprk.cpp:729-787, 808-818).
 template <int M, typename R>
-__device__ __forceinline__ void r_col(R gam, const Eig<R>& e, R* Rc) {
+__device__ __forceinline__ void r_col(R gam, const Eig<R>& e, R* Rc, const GasConst& gc) {
 #if HOWMHD_EXACT
-    const R gam2 = (gam - R(2)) / (gam - R(1));
+    (void)gam;
+    const R gam2 = R(gc.gam2);   // (gam - 2)/(gam - 1), computed on the host
 #else
+    (void)gc;
     const R gam2 = (gam - R(2)) * rcp_fast(gam - R(1));
 #endif
     const R af = e.af, as = e.as, lf = e.lf, ls = e.ls, cs = e.cs, cs2 = e.cs2;

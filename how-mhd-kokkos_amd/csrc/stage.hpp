@@ -69,6 +69,7 @@ SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     if (d == 1) { a.ct_a = (R*)c->ct[3]; a.ct_b = (R*)c->ct[2]; }
     a.g = c->g;
     a.ph = c->ph;
+    a.gc = c->gc;
     a.ct_active = c->ctf.active;
     a.mlp = c->mlp;
     a.stats = c->route_now ? c->iwave : nullptr;
@@ -76,6 +77,43 @@ SweepArgs<R> sweep_args(howmhd_gpu_ctx* c, int d, const double* qsrc) {
     a.fixlist = c->fixlist;
     a.fixctl = c->fixctl;
     return a;
+}
+
+inline StageCoef stage_coef(const howmhd_gpu_ctx* c, int iw, double dt) {
+    const RK k = rk_coeff(iw);
+    StageCoef s;
+    s.k1 = k.k1;
+    s.k3 = k.k3;
+    s.c4 = (iw == 5) ? k.k4 / k.k1 : 0.0;
+    s.c5 = (iw == 5) ? k.k5 / k.k1 : 0.0;
+    s.dtdx = k.k2 * dt / c->p.dx;   // src/ssprk.cpp:1302-1304
+    s.dtdy = k.k2 * dt / c->p.dy;
+    s.dtdz = k.k2 * dt / c->p.dz;
+    s.stage5 = (iw == 5);
+    return s;
+}
+
+// The stage combination of stage iw (its inputs: q(0), q(iw-1), stage 2 and
+// 3 at stage 5, the faces and interface fluxes of stage iw; its output q(iw)).
+template <typename R>
+CombineArgs<R> combine_args(howmhd_gpu_ctx* c, int iw, double dt) {
+    CombineArgs<R> ca{};
+    ca.q0 = c->qbuf[c->qid[0]];
+    ca.q2 = c->qbuf[c->qid[2]];
+    ca.q3 = c->qbuf[c->qid[3]];
+    ca.qp = c->qbuf[c->qid[iw - 1]];
+    ca.qo = c->qbuf[c->qid[iw]];
+    ca.bo = c->fbuf[c->qid[iw]];
+    ca.dFx = (const R*)c->dF[0];
+    ca.dFy = (const R*)c->dF[1];
+    ca.dFz = (const R*)c->dF[2];
+    ca.g = c->g;
+    ca.f = c->ctf;
+    ca.s = stage_coef(c, iw, dt);
+    ca.ph = c->ph;
+    ca.cfl = (iw == 5) ? c->cflpart : nullptr;
+    ca.floors = (iw == 5) ? c->floorpart : nullptr;
+    return ca;
 }
 
 template <typename R>
@@ -93,20 +131,8 @@ int launch_sweeps(howmhd_gpu_ctx* c, const double* qsrc) {
 
 template <typename R>
 int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
-    const RK k = rk_coeff(iw);
-    StageCoef s;
-    s.k1 = k.k1;
-    s.k3 = k.k3;
-    s.c4 = (iw == 5) ? k.k4 / k.k1 : 0.0;
-    s.c5 = (iw == 5) ? k.k5 / k.k1 : 0.0;
-    s.dtdx = k.k2 * dt / c->p.dx;   // src/ssprk.cpp:1302-1304
-    s.dtdy = k.k2 * dt / c->p.dy;
-    s.dtdz = k.k2 * dt / c->p.dz;
-    s.stage5 = (iw == 5);
-
-    const double* q0 = c->qbuf[c->qid[0]];
+    const StageCoef s = stage_coef(c, iw, dt);
     const double* qp = c->qbuf[c->qid[iw - 1]];
-    double* qo = c->qbuf[c->qid[iw]];
     const double* b0 = c->fbuf[c->qid[0]];
     const double* bp = c->fbuf[c->qid[iw - 1]];
     double* bo = c->fbuf[c->qid[iw]];
@@ -175,23 +201,8 @@ int launch_stage(howmhd_gpu_ctx* c, int iw, double dt) {
     }
     HCHECK(c, hipGetLastError());
 
-    CombineArgs<R> ca;
-    ca.q0 = q0;
-    ca.q2 = c->qbuf[c->qid[2]];
-    ca.q3 = c->qbuf[c->qid[3]];
-    ca.qp = qp;
-    ca.qo = qo;
-    ca.bo = bo;
-    ca.dFx = (const R*)c->dF[0];
-    ca.dFy = (const R*)c->dF[1];
-    ca.dFz = (const R*)c->dF[2];
-    ca.g = c->g;
-    ca.f = c->ctf;
-    ca.s = s;
-    ca.ph = c->ph;
-    ca.cfl = (iw == 5) ? c->cflpart : nullptr;
-    ca.floors = (iw == 5) ? c->floorpart : nullptr;
     {
+        const CombineArgs<R> ca = combine_args<R>(c, iw, dt);
         TimedLaunch tl(c, HOWMHD_K_COMBINE);
         hipLaunchKernelGGL((k_combine<R, EX>), dim3(nb), dim3(256), 0, c->stream, ca);
     }

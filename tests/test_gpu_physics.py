@@ -145,7 +145,9 @@ def _floor_blast():
     """MHD blast 16^3 with the default floors (rhomin = pgmin = 1e-12), the
     ambient pressure set to 1e-13 and a 3^3 pocket of density 1e-13: prot
     applies both floors (the restatement counts 1 and 1137 cells at the first
-    step's final stage)."""
+    step's final stage).  One step only: the second step drives the pocket to
+    non-numbers, in the restatement as on the GPU, and sums of those compare
+    nothing."""
     run, q0, p = _setup("mhd_blast3d", nx=16, ny=16, nz=16)
     a = H.active(q0)
     BB2 = a[4] ** 2 + a[5] ** 2 + a[6] ** 2
@@ -156,7 +158,7 @@ def _floor_blast():
 
 
 DIAG_CASES = {
-    "floors_default": (_floor_blast, 2),
+    "floors_default": (_floor_blast, 1),
     "blast_open": (lambda: _setup("mhd_blast3d", nx=12, ny=14, nz=16, x1bc="open", x3bc="open"), 2),
     "ot2d": (lambda: _setup("orszag_tang", nx=32, ny=32, nz=1, cour=0.8), 3),
     "briowu_open": (lambda: _setup("brio_wu", nx=64, ny=1, nz=1, x1bc="open", x2bc="open", x3bc="open"), 4),

@@ -8,7 +8,8 @@ periodic wraps, open copies the edge cell; an axis of one cell maps every
 ghost to it).  The expected arrays are built here with numpy index maps and
 compared bit for bit.  Shapes cover ragged tiles (sizes that are not
 multiples of the 32-cell transpose tile), 2-D and 1-D grids, and a grid with
-more than 65535 y planes (the kernels' plane loop).
+more than 65535 cells along x (ny and nz are limited to 65535 by create: they
+go into the launch grids' y and z dimensions).
 """
 import ctypes as C
 
@@ -52,7 +53,7 @@ SHAPES = [
     ((37, 5, 70), (PER, OPEN, PER)),
     ((40, 33, 1), (OPEN, PER, PER)),
     ((64, 1, 1), (OPEN, OPEN, OPEN)),
-    ((3, 65540, 1), (PER, OPEN, PER)),
+    ((65540, 3, 1), (PER, OPEN, PER)),
 ]
 
 
@@ -74,3 +75,25 @@ def test_transfer_round_trip(shape, bc, require_gpu):
         want = _expected(src, shape, bc)
         bad = np.count_nonzero(got.view(np.int64) != want.view(np.int64))
         assert bad == 0, f"{name}: {bad} values differ"
+
+
+def test_upload_right_after_create_256(require_gpu):
+    """A 256^3 context (12 GB of buffers cleared at create) takes an upload at
+    once and returns it bit for bit.  The clears run on the context's stream;
+    when they ran on the null stream, which the context's non-blocking stream
+    does not wait for, the last clears (the transfer staging buffer among
+    them) could still run during the first upload and zero part of q(0)."""
+    shape, bc = (256, 256, 256), (PER, PER, PER)
+    print(f"[transfer] shape={shape} bc={bc}, upload at once after create", flush=True)
+    rng = np.random.default_rng(7)
+    full = tuple(n + 6 for n in shape)
+    q = 1.0 + rng.random((8,) + full)
+    for trial in range(2):
+        g = H.GpuSolver(_params(*shape, bc))
+        try:
+            g.upload_q0(q)
+            qd = g.q0()
+        finally:
+            g.close()
+        bad = np.count_nonzero(H.active(qd).view(np.int64) != H.active(q).view(np.int64))
+        assert bad == 0, f"trial {trial}: {bad} values differ"
