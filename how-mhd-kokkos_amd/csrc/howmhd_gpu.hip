@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #if !HOWMHD_EXACT
@@ -154,25 +155,53 @@ double dt_from_vmax(const howmhd_params& p, const double* vmax) {
 constexpr int kMaxYZ = 65535;
 
 // Parameter checks of howmhd_gpu_create (no HIP call); "" when valid.
-std::string check_params(const howmhd_params* p) {
+std::string check_params(const howmhd_params* p, const std::string& who = "howmhd_gpu_create") {
     const int n[3] = {p->nx, p->ny, p->nz};
     for (int d = 0; d < 3; ++d) {
-        if (n[d] < 1 || n[d] == 2) return "howmhd_gpu_create: each axis needs n == 1 or n >= 3 active cells";
+        if (n[d] < 1 || n[d] == 2) return who + ": each axis needs n == 1 or n >= 3 active cells";
         if (p->bc[d] != HOWMHD_BC_PERIODIC && p->bc[d] != HOWMHD_BC_OPEN)
-            return "howmhd_gpu_create: unknown boundary condition code";
+            return who + ": unknown boundary condition code";
     }
     if (p->ny > kMaxYZ || p->nz > kMaxYZ)
-        return "howmhd_gpu_create: ny and nz must not exceed " + std::to_string(kMaxYZ) +
+        return who + ": ny and nz must not exceed " + std::to_string(kMaxYZ) +
                " (launch grid limit on the y and z axes); got ny = " + std::to_string(p->ny) +
                ", nz = " + std::to_string(p->nz);
-    if ((long long)p->nx * p->ny * p->nz > (1LL << 40)) return "howmhd_gpu_create: more than 2^40 active cells";
+    if ((long long)p->nx * p->ny * p->nz > (1LL << 40)) return who + ": more than 2^40 active cells";
     if (p->precision != HOWMHD_PREC_FP64 && p->precision != HOWMHD_PREC_FP32 && p->precision != HOWMHD_PREC_FP64_FAST)
-        return "howmhd_gpu_create: unknown precision";
+        return who + ": unknown precision";
     if (p->indicator != HOWMHD_INDICATOR_OFF && p->indicator != HOWMHD_INDICATOR_MLP)
-        return "howmhd_gpu_create: unknown indicator";
+        return who + ": unknown indicator";
     if (!(p->dx > 0 && p->dy > 0 && p->dz > 0) || !(p->gam > 1.0))
-        return "howmhd_gpu_create: dx, dy, dz must be > 0 and gam > 1";
+        return who + ": dx, dy, dz must be > 0 and gam > 1";
     return "";
+}
+
+// Scratch of howmhd_gpu_tstep_noalloc: CFL_REF_BLOCKS per-block maximum
+// triples and their fold (24.6 KB).  Each GPU the library runs on holds its
+// own copy from the first call on it until the process ends.  In host code
+// the array's name is not a device address: kernels and copies use the
+// address hipGetSymbolAddress returns on the current device.
+static __device__ __align__(256) double scratch[3 * CFL_REF_BLOCKS + 3];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on device dev (current), looked up once per
+// device; nullptr (and *err set) when the runtime refuses.
+double* scratch_on(int dev, std::string* err) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            *err = std::string("howmhd_gpu_tstep_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_dev[dev] = (double*)ptr;
+    }
+    return g_scratch_dev[dev];
 }
 
 }  // namespace
@@ -367,6 +396,52 @@ int howmhd_gpu_tstep(howmhd_gpu_ctx* c, double* dt) {
     double vmax[3];
     for (int d = 0; d < 3; ++d) std::memcpy(&vmax[d], &h[d], sizeof(double));
     *dt = dt_from_vmax(c->p, vmax);
+    return HOWMHD_OK;
+}
+
+int howmhd_gpu_tstep_noalloc(const howmhd_params* p, const double* q0_dev, void* stream, double* dt) {
+    static const char* who = "howmhd_gpu_tstep_noalloc";
+    if (!p || !q0_dev || !dt) {
+        g_create_error = std::string(who) + ": null argument";
+        return HOWMHD_EINVAL;
+    }
+    const std::string bad = check_params(p, who);
+    if (!bad.empty()) {
+        g_create_error = bad;
+        return HOWMHD_EINVAL;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        g_create_error = std::string(who) + ": no current HIP device (or device index >= 64)";
+        return HOWMHD_EHIP;
+    }
+    std::string err;
+    double* part = scratch_on(dev, &err);
+    if (!part) {
+        g_create_error = err;
+        return HOWMHD_EHIP;
+    }
+    Geom g{};
+    g.n[0] = p->nx;
+    g.n[1] = p->ny;
+    g.n[2] = p->nz;
+    g.ncell = (long long)p->nx * p->ny * p->nz;
+    Phys ph{p->gam, p->rhomin, p->pgmin};
+    const hipStream_t st = (hipStream_t)stream;
+    const unsigned nb = (unsigned)std::min<long long>((g.ncell + 255) / 256, CFL_REF_BLOCKS);
+    hipLaunchKernelGGL(k_cfl_ref, dim3(nb), dim3(256), 0, st, q0_dev, g, ph, part);
+    hipLaunchKernelGGL(k_cfl_ref_fold, dim3(1), dim3(256), 0, st, part, (int)nb);
+    double vmax[3];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(vmax, part + 3 * CFL_REF_BLOCKS, sizeof(vmax), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g_create_error = std::string(who) + ": " + hipGetErrorString(e);
+        return HOWMHD_EHIP;
+    }
+    *dt = dt_from_vmax(*p, vmax);
     return HOWMHD_OK;
 }
 

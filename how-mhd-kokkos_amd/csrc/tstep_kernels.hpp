@@ -43,6 +43,42 @@ __global__ __launch_bounds__(256) void k_cfl_fold(const double* part, long long 
         for (int d = 0; d < 3; ++d) atomicMax(&out[d], (unsigned long long)__double_as_longlong(bm[d]));
 }
 
+// The same maxima of a q(0) that the caller holds on the device in the
+// reference layout: 8 arrays of (nx+6)(ny+6)(nz+6) values, three ghost layers
+// per side, k fastest (ghosts are not read).  At most CFL_REF_BLOCKS blocks
+// walk the active cells with z fastest, so that a wavefront reads consecutive
+// k; block b leaves its maximum triple in part[3b..3b+2], and k_cfl_ref_fold
+// folds the triples of the nb blocks into part[3 CFL_REF_BLOCKS..+2].  Every
+// block writes its triple, so nothing has to be cleared before a launch.
+constexpr int CFL_REF_BLOCKS = 1024;
+
+__global__ __launch_bounds__(256) void k_cfl_ref(const double* q, Geom g, Phys ph, double* part) {
+    const long long nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const long long n6y = ny + 6, n6z = nz + 6;
+    const long long vstride = (nx + 6) * n6y * n6z;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < g.ncell; i += (long long)gridDim.x * 256) {
+        const long long z = i % nz, t = i / nz;
+        const long long y = t % ny, x = t / ny;
+        const long long o = ((x + 3) * n6y + y + 3) * n6z + z + 3;
+        double c[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) c[m] = q[m * vstride + o];
+        cfl_cell(c, ph, m0, m1, m2);
+    }
+    block_max3(m0, m1, m2, part + 3 * (long long)blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_cfl_ref_fold(double* part, int nb) {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        m0 = fmax(m0, part[3 * b]);
+        m1 = fmax(m1, part[3 * b + 1]);
+        m2 = fmax(m2, part[3 * b + 2]);
+    }
+    block_max3(m0, m1, m2, part + 3 * CFL_REF_BLOCKS);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }

@@ -107,6 +107,7 @@ GPU_FUNCS = {
     "howmhd_gpu_init_faces": (C.c_int, [C.c_void_p]),
     "howmhd_gpu_upload_faces": (C.c_int, [C.c_void_p, _DP]),
     "howmhd_gpu_tstep": (C.c_int, [C.c_void_p, _DP]),
+    "howmhd_gpu_tstep_noalloc": (C.c_int, [C.POINTER(Params), C.c_void_p, C.c_void_p, _DP]),
     "howmhd_gpu_ssprk": (C.c_int, [C.c_void_p, C.c_double]),
     "howmhd_gpu_download_q0": (C.c_int, [C.c_void_p, _DP]),
     "howmhd_gpu_download_faces": (C.c_int, [C.c_void_p, _DP]),
@@ -442,6 +443,23 @@ def mlp_adam(P, M, V, G, t: int, lr: float, beta1=0.9, beta2=0.999, eps=1e-8):
                                         eps, _stream())
     if rc:
         raise HowmhdError(f"howmhd_mlp_adam failed ({rc})")
+
+
+def tstep_noalloc(params: Params, q0_dev) -> float:
+    """howmhd_gpu_tstep_noalloc: the CFL dt of q0_dev, a contiguous float64
+    CUDA tensor of shape (8, nx+6, ny+6, nz+6) in the reference layout, on
+    torch's current stream; allocates no device memory."""
+    torch = _torch()
+    shape = (8, params.nx + 6, params.ny + 6, params.nz + 6)
+    if not (isinstance(q0_dev, torch.Tensor) and q0_dev.is_cuda and q0_dev.dtype == torch.float64
+            and q0_dev.is_contiguous() and tuple(q0_dev.shape) == shape):
+        raise HowmhdError(f"expected a contiguous float64 CUDA tensor of shape {shape}")
+    dt = C.c_double()
+    lib = gpu_lib()
+    rc = lib.howmhd_gpu_tstep_noalloc(C.byref(params), C.c_void_p(q0_dev.data_ptr()), _stream(), C.byref(dt))
+    if rc:
+        raise HowmhdError(f"howmhd_gpu_tstep_noalloc failed ({rc}): {lib.howmhd_gpu_last_error(None).decode()}")
+    return dt.value
 
 
 def build_kind() -> str:

@@ -133,6 +133,22 @@ HOWMHD_API int howmhd_gpu_upload_faces(howmhd_gpu_ctx* ctx, const double* b);
  * q(0) (src/tstep.cpp:11-136).  Synchronises. */
 HOWMHD_API int howmhd_gpu_tstep(howmhd_gpu_ctx* ctx, double* dt);
 
+/* The CFL dt of howmhd_gpu_tstep for a q(0) the caller holds in device
+ * memory, with no context and no device allocation: q0_dev is 8 x (nx+6) x
+ * (ny+6) x (nz+6) doubles in the reference layout (k fastest, ghosts not
+ * read); p gives the grid, dx/dy/dz, gam, cour and the floors; stream is a
+ * hipStream_t (NULL: the default stream), synchronised before returning.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The per-block maxima live in a fixed 24.6 KB
+ * file-scope device array (one per GPU, held until the process ends), so
+ * the result equals howmhd_gpu_tstep's bit for bit on every grid
+ * howmhd_gpu_create accepts; other grids (and more than 2^40 active cells)
+ * return HOWMHD_EINVAL before any HIP call, with the reason in
+ * howmhd_gpu_last_error(NULL).  All calls on one GPU share that array: calls
+ * that overlap on one GPU (other host threads, other streams) race. */
+HOWMHD_API int howmhd_gpu_tstep_noalloc(const howmhd_params* p, const double* q0_dev, void* stream, double* dt);
+
 /* One SSPRK(5,4) step of size dt (src/ssprk.cpp:1277-1329).  HOWMHD_EINVAL
  * for a negative or non-finite dt; HOWMHD_ESTATE before init_faces or
  * upload_faces, or with indicator mlp before set_indicator. */
@@ -163,8 +179,8 @@ HOWMHD_API int howmhd_gpu_get_timing(howmhd_gpu_ctx* ctx, howmhd_timing* t);
 /* Waits for all work queued on the context's stream. */
 HOWMHD_API int howmhd_gpu_synchronize(howmhd_gpu_ctx* ctx);
 
-/* Message of the last failing call on ctx (or of the last failing create
- * when ctx is NULL).  Never NULL. */
+/* Message of the last failing call on ctx (or, when ctx is NULL, of the
+ * last failing create or howmhd_gpu_tstep_noalloc).  Never NULL. */
 HOWMHD_API const char* howmhd_gpu_last_error(const howmhd_gpu_ctx* ctx);
 
 /* Build identification: the target and the rounding of the precision forms
