@@ -139,8 +139,9 @@ HOWMHD_API int howmhd_gpu_tstep(howmhd_gpu_ctx* ctx, double* dt);
  * read); p gives the grid, dx/dy/dz, gam, cour and the floors; stream is a
  * hipStream_t (NULL: the default stream), synchronised before returning.
  * No counterpart in the reference: for callers that cannot allocate device
- * memory (code under graph capture, tight serving loops, harnesses that
- * forbid hipMalloc).  The per-block maxima live in a fixed 24.6 KB
+ * memory (tight serving loops, harnesses that forbid hipMalloc); it
+ * synchronises its stream to return dt, so it is not itself for use inside
+ * a stream capture.  The per-block maxima live in a fixed 24.6 KB
  * file-scope device array (one per GPU, held until the process ends), so
  * the result equals howmhd_gpu_tstep's bit for bit on every grid
  * howmhd_gpu_create accepts; other grids (and more than 2^40 active cells)

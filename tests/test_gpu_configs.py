@@ -1,5 +1,8 @@
 """The reference configurations at their own sizes (needs a GPU).
 
+  * Orszag-Tang 128 x 128 to t = 0.5 through the drop-in driver: the VTS
+    output (all 8 conserved fields and the cell divB) equals, byte for byte,
+    the file the host writer makes from the CPU restatement's state;
   * 3-D circularly polarised Alfven wave, plain FP64, 32^3 / 64^3 / 128^3 /
     256^3, one period: L1 error of every conserved variable against the
     exact solution (the initial state) and the convergence orders;
@@ -16,12 +19,15 @@ restatement can run in seconds (test_gpu_parity.py); here the sizes are those
 of the configurations, and the checks are the properties their reports state.
 """
 import math
+import re
+import subprocess
 
 import numpy as np
 import pytest
 
 import eval_indicator as EV
 import howmhd as H
+import oracle_ctypes as O
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +39,43 @@ def _run(problem, **kw):
     kw.setdefault("tend", 1.0)
     run = H.parse(H.make_input(problem=problem, **kw))
     return run, H.init_problem(run), H.gpu_params(run)
+
+
+def test_orszag_tang_128_to_t05_driver_matches_restatement(tmp_path, require_gpu):
+    """Orszag-Tang 128 x 128 x 1 (gam 5/3, cour 0.8, tend 0.5, periodic)
+    through `how_mhd input.in` with one VTS output at t = 0.5, against the CPU
+    restatement run through the reference's main loop (dt from tstep, clipped
+    to the output time, src/main.cpp:165-223).  The host writer turns the
+    restatement's q(0) into a VTS file of its own; the two files are equal
+    byte for byte: rho, momentum, magnetic field, energy, the derived fields
+    and the cell-centred divB of src/output.cpp:314-347, and the time."""
+    inp = tmp_path / "input.in"
+    inp.write_text(H.make_input(problem="orszag_tang", nx=128, ny=128, nz=1, xsize=1.0, ysize=1.0, zsize=1.0,
+                                gam="1.6666666666666667", cour=0.8, tend=0.5, nt=1, output_format="vts"))
+    out = subprocess.run([str(H.DRIVER), str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    gpu_file = tmp_path / "output" / "dump000001_rank000000.vts"
+    run = H.parse(inp.read_text())
+    q0 = H.init_problem(run)
+    ref = O.RefSolver(H.gpu_params(run))
+    ref.upload_q0(q0)
+    ref.init_faces()
+    time, steps = 0.0, 0
+    while time < run.tend:
+        dt = ref.tstep()
+        if time + dt > run.tend:
+            dt = run.tend - time
+        ref.ssprk(dt)
+        time += dt
+        steps += 1
+    ref_dir = tmp_path / "ref"
+    H.write_output(run, ref.q0(), 1, time, str(ref_dir))
+    ref.close()
+    ref_file = ref_dir / "output" / "dump000001_rank000000.vts"
+    print(f"Orszag-Tang 128^2 to t = {time}: {steps} steps; {gpu_file.stat().st_size} bytes", flush=True)
+    m = re.search(r"output step = 1 time = \S+ last_dt = \S+ hydro_steps = (\d+)", out.stdout)
+    assert m and int(m.group(1)) == steps, out.stdout[-500:]
+    assert gpu_file.read_bytes() == ref_file.read_bytes()
 
 
 def test_cpaw_convergence_32_to_256(require_gpu):
