@@ -34,7 +34,9 @@ using namespace howmhd;
 
 namespace {
 
-std::string g_create_error = "no error";
+// message of the calling thread's last failing call without a context
+// (howmhd_gpu_create, howmhd_gpu_tstep_noalloc)
+thread_local std::string g_create_error = "no error";
 
 // Plain-WENO fallback of indicator mode 1: a routed step in which at least
 // this fraction of wavefronts ran the full WENO5 sends the next

@@ -181,7 +181,8 @@ HOWMHD_API int howmhd_gpu_get_timing(howmhd_gpu_ctx* ctx, howmhd_timing* t);
 HOWMHD_API int howmhd_gpu_synchronize(howmhd_gpu_ctx* ctx);
 
 /* Message of the last failing call on ctx (or, when ctx is NULL, of the
- * last failing create or howmhd_gpu_tstep_noalloc).  Never NULL. */
+ * calling thread's last failing create or howmhd_gpu_tstep_noalloc).  Never
+ * NULL. */
 HOWMHD_API const char* howmhd_gpu_last_error(const howmhd_gpu_ctx* ctx);
 
 /* Build identification: the target and the rounding of the precision forms
