@@ -560,6 +560,22 @@ struct CtFlags {
     int use_x, use_y, use_z, use_Ox, use_Oy, use_Oz, active;
 };
 
+// Tile of an x-sweep workgroup.  Workgroups are dispatched round-robin over
+// the 8 XCDs, and each XCD has its own L2.  In launch order the x-tiles of one
+// row would land on different XCDs, and each would fetch the cache lines of
+// the 3-cell stencil halo that it shares with its neighbours (1.5x the row's
+// bytes).  Here workgroups L and L + 8 (the same XCD, dispatched one after
+// the other) take neighbouring x-tiles of one row; rows beyond the last full
+// group of 8 keep the launch order.  A bijection on the grid.
+__device__ __forceinline__ void xcd_row_tiles(int& bx, int& by, int& bz) {
+    const unsigned gx = gridDim.x, gy = gridDim.y, nrow = gy * gridDim.z;
+    const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    bx = (int)blockIdx.x; by = (int)blockIdx.y; bz = (int)blockIdx.z;
+    if (gx == 1 || L >= (nrow / 8) * 8 * gx) return;
+    const unsigned a = L / 8, row = (a / gx) * 8 + L % 8;
+    bx = (int)(a % gx); by = (int)(row % gy); bz = (int)(row / gy);
+}
+
 // The sweep along DIR.  Reference rounding, DM = DIV_IEEE: the recompute
 // launch (a few blocks) takes the tiles queued by the sweep proper
 // (A.fixlist, A.fixctl[0] of them), clears their flags, and the last block
@@ -584,8 +600,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
             A.fixctl[1] = 0u;
         }
     } else {
-        const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-        sweep_tile<R, DIR, MODE, EX, DM>(A, blockIdx.x, blockIdx.y, blockIdx.z, blk);
+        int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
+        if constexpr (DIR == 0) xcd_row_tiles(bx, by, bz);
+        const long long blk = bx + (long long)gridDim.x * (by + (long long)gridDim.y * bz);
+        sweep_tile<R, DIR, MODE, EX, DM>(A, bx, by, bz, blk);
     }
 }
 
