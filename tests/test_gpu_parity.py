@@ -44,6 +44,10 @@ CASES = {
     # (periodic) or clamp (open) at both ends at once
     "flat3_per": ("mpi_smooth3d", dict(nx=12, ny=10, nz=3), 2),
     "flat3_open": ("mhd_blast3d", dict(nx=10, ny=9, nz=3, x3bc="open"), 2),
+    # three x-tiles per row and 9 rows: the x sweep's XCD-aware tile order
+    # on its first 8 rows, launch order on the ninth; open x: 131 interfaces
+    "wide130": ("mpi_smooth3d", dict(nx=130, ny=9, nz=3), 2),
+    "wide130_open": ("mhd_blast3d", dict(nx=130, ny=9, nz=3, x1bc="open"), 2),
 }
 
 
