@@ -18,7 +18,11 @@ cells).  Forms:
   fp64+indicator       the trained network on the FP64 method
 
 Speed: zone-cycles/s of each form over --time-steps steps from the initial
-state (the same state for every form), timed with the device synchronised.
+state (the same state for every form), timed with the device synchronised;
+with --speed-end also from the reference run's final q(0) and faces (the
+developed flow: zone_cycles_per_s_end, speedup_vs_fp64_end).  The accuracy
+pass over the whole dt sequence is timed too (ms_per_step_run,
+speedup_vs_fp64_run; no CFL reduction in that loop).
 """
 from __future__ import annotations
 
@@ -42,6 +46,9 @@ CASES = {
     "rotor512": ("mhd_rotor", dict(nx=512, ny=512, nz=1, gam=1.4, cour=0.8), dict(tend=0.15)),
     "turb256": ("turbulence3d", dict(nx=256, ny=256, nz=256, cour=0.4, seed=1), dict(steps=50)),
     "blast256": ("mhd_blast3d", dict(nx=256, ny=256, nz=256, cour=0.4), dict(steps=50)),
+    # the rotor at 16x the cells: sweep launches of many dispatch rounds
+    # instead of about one
+    "rotor2048": ("mhd_rotor", dict(nx=2048, ny=2048, nz=1, gam=1.4, cour=0.8), dict(tend=0.15)),
     # small versions (tests, quick looks)
     "rotor128": ("mhd_rotor", dict(nx=128, ny=128, nz=1, gam=1.4, cour=0.8), dict(tend=0.05)),
     "turb64": ("turbulence3d", dict(nx=64, ny=64, nz=64, cour=0.4, seed=1), dict(steps=30)),
@@ -95,7 +102,7 @@ def matched_threshold(ref_run, P_trained, thr_trained, P_other):
     return float(np.clip(thr, 1e-6, 1 - 1e-6)), frac
 
 
-def run_case(name, time_steps=10, forms=None):
+def run_case(name, time_steps=10, forms=None, speed_end=False):
     import torch
     problem, kw, stop = CASES[name]
     kw = dict(kw)
@@ -133,6 +140,7 @@ def run_case(name, time_steps=10, forms=None):
     if thr_un is None:
         thr_un, frac_cal = matched_threshold(ref_run, P, thr, P_un)
     q_ref = ref_run.q0()
+    b_ref = ref_run.faces() if speed_end else None
     ref_run.close()
     log(f"[{name}] reference: {len(dts)} steps to t = {t:.5g}")
 
@@ -150,16 +158,9 @@ def run_case(name, time_steps=10, forms=None):
     }
     out = {"case": name, "problem": problem, "input": kw, "steps": len(dts), "t_end": t, "zones": zones,
            "untrained_threshold": thr_un, "calibration_flagged_fraction": frac_cal, "forms": {}}
-    base_speed = None
-    for form in forms:
-        prec, Pf, thf = spec[form]
-        # the product forms route with the plain-WENO fallback; the ablation
-        # forms route every step
-        mode = 1 if form in ("fp32+indicator", "fp64+indicator") else 2
-        # speed from the initial state
-        g = _solver(run, prec, Pf, thf, enable=mode)
-        g.upload_q0(q0)
-        g.init_faces()
+    base_speed, base_end, base_run = None, None, None
+
+    def timed(g):
         g.ssprk(g.tstep())
         g.synchronize()
         torch.cuda.synchronize()
@@ -167,18 +168,42 @@ def run_case(name, time_steps=10, forms=None):
         for _ in range(time_steps):
             g.ssprk(g.tstep())
         g.synchronize()
-        el = time.perf_counter() - t0
+        return time.perf_counter() - t0
+    for form in forms:
+        prec, Pf, thf = spec[form]
+        # the product forms route with the plain-WENO fallback; the ablation
+        # forms route every step
+        mode = 1 if form in ("fp32+indicator", "fp64+indicator") else 2
+        # speed from the initial state (and from the reference's final state)
+        g = _solver(run, prec, Pf, thf, enable=mode)
+        speed_e, st_e = None, None
+        if speed_end:
+            g.upload_q0(q_ref)
+            g.upload_faces(b_ref)
+            speed_e = zones * time_steps / timed(g)
+            if Pf is not None:
+                st_e = g.indicator_stats()
+                g.set_indicator(Pf, thf, enable=mode)   # clears the counters and the fallback state
+        g.upload_q0(q0)
+        g.init_faces()
+        el = timed(g)
         speed = zones * time_steps / el
         if form == "fp64":
-            base_speed = speed
+            base_speed, base_end = speed, speed_e
         # accuracy along the reference run's dt sequence (same routing mode)
         if Pf is not None:
             g.set_indicator(Pf, thf, enable=mode)
         g.upload_q0(q0)
         g.init_faces()
         ok = True
+        g.synchronize()
+        t0 = time.perf_counter()
         for dt in dts:
             g.ssprk(dt)
+        g.synchronize()
+        el_run = time.perf_counter() - t0
+        if form == "fp64":
+            base_run = el_run
         qf = g.q0()
         if not np.all(np.isfinite(H.active(qf))):
             ok = False
@@ -188,6 +213,18 @@ def run_case(name, time_steps=10, forms=None):
                "divb_ct_max": g.diag().max_divb_ct}
         if base_speed:
             ent["speedup_vs_fp64"] = round(speed / base_speed, 3)
+        # the whole run along the reference's dt sequence (no tstep)
+        ent["ms_per_step_run"] = round(el_run / len(dts) * 1e3, 3)
+        if base_run:
+            ent["speedup_vs_fp64_run"] = round(base_run / el_run, 3)
+        if speed_e is not None:
+            ent["zone_cycles_per_s_end"] = round(speed_e, 1)
+            ent["ms_per_step_end"] = round(zones / speed_e * 1e3, 3)
+            if base_end:
+                ent["speedup_vs_fp64_end"] = round(speed_e / base_end, 3)
+        if st_e is not None:
+            ent["flagged_fraction_end"] = round(st_e["flagged_fraction"], 4)
+            ent["wavefront_flagged_fraction_end"] = round(st_e["wave_fraction"], 4)
         if Pf is not None:
             st = g.indicator_stats()
             ent["flagged_fraction"] = round(st["flagged_fraction"], 4)
@@ -205,13 +242,15 @@ def main(argv=None):
     ap.add_argument("--cases", default="rotor512,turb256,blast256")
     ap.add_argument("--time-steps", type=int, default=10)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--speed-end", action="store_true",
+                    help="also time every form from the reference run's final state")
     a = ap.parse_args(argv)
     import torch
     if not torch.cuda.is_available():
         raise SystemExit("eval_indicator.py needs a GPU")
     rep = {"weights": str(WEIGHTS.name), "cases": {}}
     for c in a.cases.split(","):
-        rep["cases"][c] = run_case(c, a.time_steps)
+        rep["cases"][c] = run_case(c, a.time_steps, speed_end=a.speed_end)
     if a.out:
         Path(a.out).write_text(json.dumps(rep, indent=1))
     return rep

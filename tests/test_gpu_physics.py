@@ -291,8 +291,9 @@ def test_driver_max_steps(tmp_path, require_gpu):
 
 def test_fp32_turbulence_384_stays_close_to_fp64(require_gpu):
     """Decaying turbulence at 384^3 for 100 steps (the precision-form
-    configuration): the FP32 form stays finite and within density relative L1
-    1e-3 of the plain FP64 path.  (Regression: with FP32 interface wave speeds
+    configuration): the FP32 form stays finite, within density and energy
+    relative L1 1e-3 of the plain FP64 path, and its kinetic and magnetic
+    energies within 1e-6 of FP64's.  (Regression: with FP32 interface wave speeds
     and eigenvector normalisations this run went non-finite near step 84.)"""
     run, q0, p64 = _setup("turbulence3d", nx=384, ny=384, nz=384, cour=0.4, seed=1)
     p32 = H.gpu_params(run)
@@ -310,9 +311,23 @@ def test_fp32_turbulence_384_stays_close_to_fp64(require_gpu):
             print(f"[turb384] step {step + 1}", flush=True)
     d32 = g32.diag()
     assert np.isfinite(d32.sum_cons[7]) and d32.min_rho > 0.5
-    l1 = H.rel_l1(g32.q0(), g64.q0())
-    print("turbulence 384^3 FP32 rel L1:", l1, "divB_CT", d32.max_divb_ct)
+    q32, q64 = g32.q0(), g64.q0()
+    l1 = H.rel_l1(q32, q64)
+
+    def energies(q):
+        a = H.active(q)
+        kin = 0.5 * float(np.sum((a[1] ** 2 + a[2] ** 2 + a[3] ** 2) / a[0]))
+        mag = 0.5 * float(np.sum(a[4] ** 2 + a[5] ** 2 + a[6] ** 2))
+        return kin, mag
+
+    (k32, m32), (k64, m64) = energies(q32), energies(q64)
+    ek, em = abs(k32 - k64) / k64, abs(m32 - m64) / m64
+    print("turbulence 384^3 FP32 rel L1:", l1, "kinetic / magnetic energy rel err:", ek, em,
+          "divB_CT", d32.max_divb_ct)
     assert l1[0] <= 1e-3 and l1[7] <= 1e-3
+    # the configuration's three reported errors: density, total and magnetic
+    # energy (measured: ~1e-9 and below)
+    assert ek <= 1e-6 and em <= 1e-6
     assert d32.max_divb_ct <= 1e-10
 
 
