@@ -230,7 +230,6 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     }
     c->ncell = (size_t)n[0] * n[1] * n[2];
     c->g.ncell = (long long)c->ncell;
-    c->g.sx = 1; c->g.sy = n[0]; c->g.sz = n[0] * n[1];
     c->ph.gam = p->gam; c->ph.rhomin = p->rhomin; c->ph.pgmin = p->pgmin;
     // IEEE quotients on the host: the bits the device's divisions give
     c->gc.gam2 = (p->gam - 2.0) / (p->gam - 1.0);

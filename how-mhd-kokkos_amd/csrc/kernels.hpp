@@ -39,7 +39,6 @@ namespace howmhd {
 struct Geom {
     int n[3];      // active cells per axis
     int per[3];    // 1 periodic, 0 open
-    int sx, sy, sz;
     long long ncell;
 };
 
