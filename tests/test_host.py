@@ -338,3 +338,30 @@ def test_bench_refuses_more_than_one_gpu():
                          timeout=120)
     assert out.returncode == 2
     assert "measures one GPU" in out.stderr and out.stdout == ""
+
+
+# ------------------------------------------------------------------ driver
+DRIVER = ROOT / "how-mhd-kokkos_amd" / "bin" / "how_mhd"
+
+
+@pytest.mark.skipif(not DRIVER.exists(), reason="driver not built")
+@pytest.mark.parametrize("text,msg", [
+    (None, "Error: cannot open input file"),                                    # src/main.cpp:64-79
+    ("nx 0 ny 1 nz 1 nt 1 tend 1\n", "global grid size must be positive"),      # parameters.cpp:121-137
+    ("nx 4 ny 4 nz 4 nt 1 tend 1 problem nope\n", "unknown problem = nope"),    # problem.cpp:24-34
+    ("nx 4 ny 4 nz 4 nt 1 tend 1 problem uniform output_format xyz\n",
+     "Error: unknown output_format = xyz"),                                     # output.cpp:565-569
+    ("nx 4 ny 4 nz 4 nt 1 tend 1 problem uniform x3bc reflecting\n", "unknown x3bc = reflecting"),  # bound.cpp:305-315
+])
+def test_driver_aborts_on_bad_input(tmp_path, text, msg):
+    """The drop-in driver stops with the reference's message and an abort
+    (SIGABRT) on each input error, before it touches a GPU."""
+    import signal
+    import subprocess
+    path = tmp_path / "input.in"
+    if text is not None:
+        path.write_text(text)
+    r = subprocess.run([str(DRIVER), str(path)], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == -signal.SIGABRT, (r.returncode, r.stdout[-500:], r.stderr[-500:])
+    assert msg in r.stderr, r.stderr[-1000:]
+    assert not (tmp_path / "output").exists()
