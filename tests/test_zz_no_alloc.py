@@ -4,8 +4,9 @@ memory, against howmhd_gpu_tstep (needs a GPU).
 Two calls in a row, the second on a smaller grid with smaller signal speeds:
 each dt must equal, bit for bit, the dt that howmhd_gpu_tstep gives for the
 same state (the shared scratch keeps the first call's per-block maxima, which
-the second call must neither read nor need cleared).  Bad arguments return
-HOWMHD_EINVAL before any HIP call.
+the second call must neither read nor need cleared); a third call repeats the
+second on a stream of the caller's.  Bad arguments return HOWMHD_EINVAL before
+any HIP call.
 """
 import ctypes as C
 
@@ -78,6 +79,14 @@ def test_tstep_noalloc_matches_tstep(require_gpu):
     # the second grid's speeds are lower, so a stale maximum of the first
     # call would have shortened its dt
     assert speeds[1] < speeds[0], speeds
+    # a third call on a stream of the caller's (the upload and the reduction
+    # both queued there)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        qd = torch.from_numpy(q).cuda()
+        got = H.tstep_noalloc(p, qd)
+    side.synchronize()
+    assert got == dt, ("side stream", got, dt)
 
 
 def test_tstep_noalloc_rejects_bad_arguments(require_gpu):
