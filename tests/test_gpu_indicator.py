@@ -254,6 +254,41 @@ def test_trained_indicator_ablation(require_gpu):
     assert (not un["finite"]) or un["rel_l1_rho"] >= 10 * tr["rel_l1_rho"]
 
 
+def test_training_script_learns_the_labels(tmp_path, require_gpu):
+    """train_indicator.py end to end at 1/8 of its grids (Orszag-Tang 32^2,
+    blast and turbulence 8^3): samples collected by the FP64 sweep, 4 epochs
+    on the library's gradient and Adam kernels, threshold for held-out recall
+    0.99.  The network must beat a flagger that knows only the positive rate
+    (its precision would equal that rate), and the weights file it writes must
+    load through the host library and route a step."""
+    import train_indicator as TR
+    out = tmp_path / "w.txt"
+    st = TR.main(["--out", str(out), "--scale", "0.125", "--samples", "4e5", "--epochs", "4",
+                  "--batch", "2048", "--recall", "0.99", "--seed", "3"])
+    print("held-out:", st, flush=True)
+    assert st["samples"] >= 1000
+    assert 0.05 < st["positive_fraction"] < 0.95
+    assert st["recall"] >= 0.98
+    assert st["precision"] >= st["positive_fraction"] + 0.1
+    assert st["flagged_fraction"] < 0.95
+    P, thr = H.load_indicator(out)
+    assert P.shape == (H.MLP_NPARAM,) and np.all(np.isfinite(P)) and abs(thr - st["threshold"]) < 1e-6
+    run = H.parse(H.make_input(problem="orszag_tang", nx=32, ny=32, nz=1, cour=0.8, nt=1, tend=1.0))
+    p = H.gpu_params(run)
+    p.precision = H.PREC["fp32"]
+    g = H.GpuSolver(p)
+    try:
+        g.set_indicator(P, thr, enable=2)
+        g.upload_q0(H.init_problem(run))
+        g.init_faces()
+        for _ in range(3):
+            g.ssprk(g.tstep())
+        assert np.all(np.isfinite(H.active(g.q0())))
+        assert g.indicator_stats()["total"] > 0
+    finally:
+        g.close()
+
+
 def test_plain_weno_fallback(require_gpu):
     """Routing mode 1: where nearly every wavefront is flagged (decaying
     turbulence) the steps fall back to the plain sweep, re-checking every 9th
