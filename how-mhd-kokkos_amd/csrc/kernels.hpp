@@ -97,9 +97,11 @@ template <> struct SweepTile<2> {
 #endif
 // Stencil cells whose LDS reads the scheduler may overlap; a scheduling
 // barrier after every such batch keeps it from hoisting all 42 reads of a
-// group ahead of the arithmetic (which spills registers).
+// pass ahead of the arithmetic (which spills registers).  One cell per batch:
+// the FP64 sweeps then need 182-212 registers and the FP32 sweeps 93-113,
+// none spilled; two cells per batch spill in every form.
 #ifndef HOWMHD_PROJ_BATCH
-#define HOWMHD_PROJ_BATCH 3
+#define HOWMHD_PROJ_BATCH 1
 #endif
 
 __host__ __device__ inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -207,7 +209,7 @@ __device__ __forceinline__ void opaque_after(Eig<R>& o, R a) {
     opq_after(o.vx, a); opq_after(o.vy, a); opq_after(o.vz, a); opq_after(o.vv2, a);
     opq_after(o.cs, a); opq_after(o.cs2, a); opq_after(o.lf, a); opq_after(o.ls, a);
     opq_after(o.af, a); opq_after(o.as, a); opq_after(o.bty, a); opq_after(o.btz, a);
-    opq_after(o.sgnBx, a); opq_after(o.invsqrtrho, a); opq_after(o.sgnBt, a);
+    opq_after(o.sgnBx, a); opq_after(o.invsqrtrho, a);
 }
 
 // Makes every scalar of e opaque to the optimiser (no instruction emitted).
@@ -215,14 +217,9 @@ template <typename R>
 __device__ __forceinline__ void opaque(Eig<R>& o) {
     opq(o.vx); opq(o.vy); opq(o.vz); opq(o.By); opq(o.Bz); opq(o.vv2); opq(o.cs); opq(o.cs2);
     opq(o.lf); opq(o.ls); opq(o.af); opq(o.as); opq(o.bty); opq(o.btz); opq(o.sgnBx);
-    opq(o.sqrtrho); opq(o.invsqrtrho); opq(o.sgnBt); opq(o.nrm_half); opq(o.nrm_ent);
+    opq(o.sqrtrho); opq(o.invsqrtrho); opq(o.nrm_half); opq(o.nrm_ent);
 }
 
-// The seven per-wave LLF speeds, accumulated as named registers.
-template <typename R>
-struct Seven {
-    R v0, v1, v2, v3, v4, v5, v6;
-};
 // Characteristic LLF-WENO5 flux of the interface of phase-2 thread (row,
 // lane) from the sweep tile in LDS (src/ssprk.cpp:825-946): interface
 // eigensystem, the 7 wave maxima, then the waves in pairs, added to dFo in
@@ -254,38 +251,62 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
     // and takes unscaled square roots (operands 0 or >= 2^-767)
     if constexpr (DM == DIV_UNSCALED) bad |= __ballot(!(e.nrm_half > R(0)) || sqbad);
 
-    // largest |eigenvalue| of each wave over the 6 cells (src/ssprk.cpp:851-854)
-    Seven<R> amx{R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+    // The waves are projected in four passes over the stencil in LDS,
+    // (0,6), (1,5), (2,4) and (3), and dF still gets them added in order 0..6
+    // (the fluxes of waves 4, 5, 6 wait for wave 3).  A pass shares work
+    // between its two rows of L (src/ssprk.cpp:662-716):
+    //  * rows 0 and 6, and rows 2 and 4, have equal entries 4..6, so those
+    //    three products of a stencil cell serve both rows;
+    //  * rows 1 and 5 have equal entries 0, 2, 3, opposite entries 4, 5 and
+    //    zeros at 1 and 6: both sums share the partial sum over entries 0, 2,
+    //    3, row 5 subtracts row 1's last two products, and the zero products
+    //    are left out.
+    // A product left out is +-0 and a product reused with its sign changed
+    // is exact, so a projected value can differ from the reference's only in
+    // the sign of a zero.  The wave's flux then differs at most in the sign
+    // of a zero, and a zero product added to dF leaves it unchanged (dF
+    // starts at +0; a sum of doubles is -0 only when both addends are).  For
+    // the same reason the structural zeros of R (columns 1 and 5 in rows 0,
+    // 1; column 3 in rows 4, 5) add nothing and are skipped.
+    auto add_col = [&](auto mc, const R* Rc, R Fs) {
+        constexpr int M = decltype(mc)::value;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const int s = slot(k);
-        const R vn = lo[O_VN][s], lf = lo[O_LF][s], la = lo[O_LA][s], ls = lo[O_LS][s];
-        amx.v0 = rmax(amx.v0, rabs(vn - lf));
-        amx.v1 = rmax(amx.v1, rabs(vn - la));
-        amx.v2 = rmax(amx.v2, rabs(vn - ls));
-        amx.v3 = rmax(amx.v3, rabs(vn));
-        amx.v4 = rmax(amx.v4, rabs(vn + ls));
-        amx.v5 = rmax(amx.v5, rabs(vn + la));
-        amx.v6 = rmax(amx.v6, rabs(vn + lf));
-    }
-
-    // Waves are taken in pairs (0,1), (2,3), (4,5), (6): both waves of a pair
-    // are projected in one pass over the stencil in LDS, and dF still gets
-    // the waves added in order 0..6.
-    const R amxv[7] = {amx.v0, amx.v1, amx.v2, amx.v3, amx.v4, amx.v5, amx.v6};
+        for (int r = 0; r < 7; ++r) {
+            if ((M == 1 || M == 5) && r < 2) continue;
+            if (M == 3 && (r == 4 || r == 5)) continue;
+            dFo[r] += Rc[r] * Fs;
+        }
+    };
+    R Fs4 = R(0), Fs5 = R(0), Fs6 = R(0);
     sfor<0, 4>([&](auto pc) {
-        constexpr int m0 = 2 * decltype(pc)::value;
-        constexpr bool two = (m0 + 1 < 7);
-        constexpr int m1 = two ? m0 + 1 : m0;
-        // the stencil is read from LDS again for every pair rather than kept
+        constexpr int P = decltype(pc)::value;
+        constexpr int ma = P;                // 0, 1, 2, 3
+        constexpr int mb = 6 - P;            // 6, 5, 4 (3: none)
+        constexpr bool two = (P < 3);
+        // the stencil is read from LDS again for every pass rather than kept
         // live in registers
         asm volatile("" ::: "memory");
-        // the scalars become opaque here, so the compiler builds this pair's
+        // the scalars become opaque here, so the compiler builds this pass's
         // rows and columns here instead of all seven up front (98 live values)
         opaque(e);
+        // largest |eigenvalue| of the pass's waves over the 6 cells
+        // (src/ssprk.cpp:851-854): v_n -+ c_f, v_n -+ c_a, v_n -+ c_s, v_n
+        R amxa = R(0), amxb = R(0);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int s = slot(k);
+            const R vn = lo[O_VN][s];
+            if constexpr (P == 3) {
+                amxa = rmax(amxa, rabs(vn));
+            } else {
+                const R c = lo[P == 0 ? O_LF : P == 1 ? O_LA : O_LS][s];
+                amxa = rmax(amxa, rabs(vn - c));
+                amxb = rmax(amxb, rabs(vn + c));
+            }
+        }
         R La[7], Lb[7];
-        l_row<m0, R, DM>(gam, e, La, gc);
-        if constexpr (two) l_row<m1, R, DM>(gam, e, Lb, gc);
+        l_row<ma, R, DM>(gam, e, La, gc);
+        if constexpr (P == 0 || P == 2) l_row<mb, R, DM>(gam, e, Lb, gc);
         R Fa[6], qa[6], Fb[6], qb[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -297,32 +318,51 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
                 U[j] = v.u;
                 F[j] = v.f;
             }
-            Fa[k] = La[0] * F[0] + La[1] * F[1] + La[2] * F[2] + La[3] * F[3] + La[4] * F[4] + La[5] * F[5] + La[6] * F[6];
-            qa[k] = La[0] * U[0] + La[1] * U[1] + La[2] * U[2] + La[3] * U[3] + La[4] * U[4] + La[5] * U[5] + La[6] * U[6];
-            if constexpr (two) {
-                Fb[k] = Lb[0] * F[0] + Lb[1] * F[1] + Lb[2] * F[2] + Lb[3] * F[3] + Lb[4] * F[4] + Lb[5] * F[5] + Lb[6] * F[6];
-                qb[k] = Lb[0] * U[0] + Lb[1] * U[1] + Lb[2] * U[2] + Lb[3] * U[3] + Lb[4] * U[4] + Lb[5] * U[5] + Lb[6] * U[6];
+            if constexpr (P == 1) {
+                const R pf = La[0] * F[0] + La[2] * F[2] + La[3] * F[3];
+                const R pu = La[0] * U[0] + La[2] * U[2] + La[3] * U[3];
+                const R f4 = La[4] * F[4], f5 = La[5] * F[5];
+                const R u4 = La[4] * U[4], u5 = La[5] * U[5];
+                Fa[k] = pf + f4 + f5;
+                qa[k] = pu + u4 + u5;
+                Fb[k] = pf - f4 - f5;
+                qb[k] = pu - u4 - u5;
+            } else if constexpr (two) {
+                const R f4 = La[4] * F[4], f5 = La[5] * F[5], f6 = La[6] * F[6];
+                const R u4 = La[4] * U[4], u5 = La[5] * U[5], u6 = La[6] * U[6];
+                Fa[k] = La[0] * F[0] + La[1] * F[1] + La[2] * F[2] + La[3] * F[3] + f4 + f5 + f6;
+                qa[k] = La[0] * U[0] + La[1] * U[1] + La[2] * U[2] + La[3] * U[3] + u4 + u5 + u6;
+                Fb[k] = Lb[0] * F[0] + Lb[1] * F[1] + Lb[2] * F[2] + Lb[3] * F[3] + f4 + f5 + f6;
+                qb[k] = Lb[0] * U[0] + Lb[1] * U[1] + Lb[2] * U[2] + Lb[3] * U[3] + u4 + u5 + u6;
+            } else {
+                Fa[k] = La[0] * F[0] + La[1] * F[1] + La[2] * F[2] + La[3] * F[3] + La[4] * F[4] + La[5] * F[5] + La[6] * F[6];
+                qa[k] = La[0] * U[0] + La[1] * U[1] + La[2] * U[2] + La[3] * U[3] + La[4] * U[4] + La[5] * U[5] + La[6] * U[6];
             }
             if ((k + 1) % HOWMHD_PROJ_BATCH == 0) asm volatile("" ::: "memory");
         }
         R Fsa, Fsb = R(0);
         if constexpr (MODE == SW_COLLECT) {
-            Fsa = weno_wave_dev<R, DM>(Fa, qa, amxv[m0], mdev, bad);
-            if constexpr (two) Fsb = weno_wave_dev<R, DM>(Fb, qb, amxv[m1], mdev, bad);
+            Fsa = weno_wave_dev<R, DM>(Fa, qa, amxa, mdev, bad);
+            if constexpr (two) Fsb = weno_wave_dev<R, DM>(Fb, qb, amxb, mdev, bad);
         } else {
-            Fsa = weno_wave<R, DM>(Fa, qa, amxv[m0], bad);
-            if constexpr (two) Fsb = weno_wave<R, DM>(Fb, qb, amxv[m1], bad);
+            Fsa = weno_wave<R, DM>(Fa, qa, amxa, bad);
+            if constexpr (two) Fsb = weno_wave<R, DM>(Fb, qb, amxb, bad);
         }
         opaque_after(e, Fsa);
         if constexpr (two) opaque_after(e, Fsb);
         R Rc[7];
-        r_col<m0>(gam, e, Rc, gc);
-#pragma unroll
-        for (int r = 0; r < 7; ++r) dFo[r] += Rc[r] * Fsa;
-        if constexpr (two) {
-            r_col<m1>(gam, e, Rc, gc);
-#pragma unroll
-            for (int r = 0; r < 7; ++r) dFo[r] += Rc[r] * Fsb;
+        r_col<ma>(gam, e, Rc, gc);
+        add_col(std::integral_constant<int, ma>{}, Rc, Fsa);
+        if constexpr (P == 0) Fs6 = Fsb;
+        else if constexpr (P == 1) Fs5 = Fsb;
+        else if constexpr (P == 2) Fs4 = Fsb;
+        else {
+            r_col<4>(gam, e, Rc, gc);
+            add_col(std::integral_constant<int, 4>{}, Rc, Fs4);
+            r_col<5>(gam, e, Rc, gc);
+            add_col(std::integral_constant<int, 5>{}, Rc, Fs5);
+            r_col<6>(gam, e, Rc, gc);
+            add_col(std::integral_constant<int, 6>{}, Rc, Fs6);
         }
     });
 }
@@ -521,21 +561,21 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
     for (int c = 0; c < 7; ++c)
         if (!(A.ct_active && (c == 4 || c == 5))) A.dF[dF_index<DIR>(g, c, t, x, y, z)] = dF[c];
 
-    // CT flux terms, src/ssprk.cpp:967-994 (cells k = 1..4; the -0.0 weights
-    // are kept as written)
+    // CT flux terms, src/ssprk.cpp:967-994: (-0 a + 9 b + 9 c - 0 d)/18 over
+    // cells k = 1..4.  The products with the zero weights are +-0 and are
+    // left out: the sum can then differ only in the sign of a zero, which
+    // adding it to dF[4] or dF[5] (never -0) does not show.
     if (t >= 1) {
-        R bnv1[4], bnv2[4];
+        R bnv1[2], bnv2[2];
 #pragma unroll
-        for (int k = 1; k <= 4; ++k) {
+        for (int k = 2; k <= 3; ++k) {
             const int s = slot(k);
             const R bn = lo[O_BN][s];
-            bnv1[k - 1] = bn * lo[O_VT1][s];
-            bnv2[k - 1] = bn * lo[O_VT2][s];
+            bnv1[k - 2] = bn * lo[O_VT1][s];
+            bnv2[k - 2] = bn * lo[O_VT2][s];
         }
-        const R ftmp1 = div_c<R>(R(-0.0) * bnv1[0] + R(9) * bnv1[1] + R(9) * bnv1[2] - R(0) * bnv1[3], R(18),
-                                 R(1.0 / 18.0));
-        const R ftmp2 = div_c<R>(R(-0.0) * bnv2[0] + R(9) * bnv2[1] + R(9) * bnv2[2] - R(0) * bnv2[3], R(18),
-                                 R(1.0 / 18.0));
+        const R ftmp1 = div_c<R>(R(9) * bnv1[0] + R(9) * bnv1[1], R(18), R(1.0 / 18.0));
+        const R ftmp2 = div_c<R>(R(9) * bnv2[0] + R(9) * bnv2[1], R(18), R(1.0 / 18.0));
         long long ci;
         if constexpr (DIR == 0) ci = ((long long)z * g.n[1] + y) * g.n[0] + (t - 1);
         else if constexpr (DIR == 1) ci = ((long long)z * g.n[1] + (t - 1)) * g.n[0] + x;

@@ -314,11 +314,21 @@ struct GasConst {
 
 // Scalars of the interface eigensystem (src/ssprk.cpp:610-659).  Rows of L
 // and columns of R are built from them one wave at a time.
+//
+// The sign flip of src/ssprk.cpp:789-818 (rows 2, 4 of L and columns 2, 4 of
+// R times sign(B_t) when c_s >= c_a, else rows and columns 0, 6) is not
+// applied: it cancels exactly.  Negating a row of L negates the wave's six
+// projected values exactly (rounding to nearest is symmetric), the WENO flux
+// of the wave is odd in them (the smoothness indicators square the
+// differences), so its flux F_s is negated exactly, and the flipped column
+// times the flipped flux is the unflipped product.  The only difference is
+// the sign of a zero F_s, and adding a zero product to a dF component leaves
+// it unchanged, because dF starts at +0 and a sum of doubles is -0 only when
+// both addends are.
 template <typename R>
 struct Eig {
-    R vx, vy, vz, By, Bz, vv2, cs, cs2, lf, ls, af, as, bty, btz, sgnBx, sqrtrho, invsqrtrho, sgnBt;
+    R vx, vy, vz, By, Bz, vv2, cs, cs2, lf, ls, af, as, bty, btz, sgnBx, sqrtrho, invsqrtrho;
     R nrm_half, nrm_ent;  // FMA form: 0.5/cs2 and (gam-1)/cs2; reference rounding: nrm_half = RN(1/cs2)
-    bool slow_flip;  // cs >= la: waves 2 and 4 take the sign of Bt; else waves 0 and 6
 };
 
 // FP32 form: fast/slow speeds and the alpha_f, alpha_s normalisations of
@@ -414,7 +424,6 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
     }
     e.af = af; e.as = as;
     }
-    const R la = rsqrt_(rmax(R(0), bnx2));
     R bty, btz;
     if (Bt2 >= R(1.0e-30)) {
 #if HOWMHD_EXACT
@@ -440,8 +449,6 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
 #else
     e.invsqrtrho = rcp_fast(sqrtrho);
 #endif
-    e.sgnBt = (By != R(0)) ? sign1(By) : sign1(Bz);
-    e.slow_flip = (cs >= la);
 #if !HOWMHD_EXACT
     const R ics2 = rcp_fast(cs2);
     e.nrm_half = R(0.5) * ics2;
@@ -453,8 +460,7 @@ __device__ __forceinline__ void eig_scalars(R gam, const R* u0, const R* u1, Eig
 #endif
 }
 
-// Row m of L after normalisation and sign flip (src/ssprk.cpp:662-726,
-// 789-806).
+// Row m of L after normalisation (src/ssprk.cpp:662-726).
 // DM: division mode of the reference-rounding normalisation (div_q; the
 // unscaled mode needs e.nrm_half > 0, which the caller checks).
 template <int M, typename R, int DM = DIV_IEEE>
@@ -542,21 +548,10 @@ __device__ __forceinline__ void l_row(R gam, const Eig<R>& e, R* L, const GasCon
         else L[c] = L[c] * e.nrm_half;
 #endif
     }
-    // sign of the transverse field, src/ssprk.cpp:796-806
-    if constexpr (M == 2 || M == 4) {
-        if (e.slow_flip) {
-#pragma unroll
-            for (int c = 0; c < 7; ++c) L[c] *= e.sgnBt;
-        }
-    } else if constexpr (M == 0 || M == 6) {
-        if (!e.slow_flip) {
-#pragma unroll
-            for (int c = 0; c < 7; ++c) L[c] *= e.sgnBt;
-        }
-    }
+    // (the sign flip of src/ssprk.cpp:796-806 cancels against that of R; see Eig)
 }
 
-// Column m of R after the sign flip (src/ssprk.cpp:729-787, 808-818).
+// Column m of R (src/ssprk.cpp:729-787).
 template <int M, typename R>
 __device__ __forceinline__ void r_col(R gam, const Eig<R>& e, R* Rc, const GasConst& gc) {
 #if HOWMHD_EXACT
@@ -626,17 +621,7 @@ __device__ __forceinline__ void r_col(R gam, const Eig<R>& e, R* Rc, const GasCo
         Rc[5] = cs * as * btz * isr;
         Rc[6] = af * (lf * lf + lf * vx + R(0.5) * vv2 - gam2 * cs2) - as * ls * (bty * vy + btz * vz) * sgnBx;
     }
-    if constexpr (M == 2 || M == 4) {
-        if (e.slow_flip) {
-#pragma unroll
-            for (int r = 0; r < 7; ++r) Rc[r] *= e.sgnBt;
-        }
-    } else if constexpr (M == 0 || M == 6) {
-        if (!e.slow_flip) {
-#pragma unroll
-            for (int r = 0; r < 7; ++r) Rc[r] *= e.sgnBt;
-        }
-    }
+    // (the sign flip of src/ssprk.cpp:808-818 cancels against that of L; see Eig)
 }
 
 // ------------------------------------------------------------------ WENO5
@@ -672,8 +657,10 @@ __device__ __forceinline__ R weno_side(R a, R b, R c, R d, unsigned long long& b
         omega2 = alpha2 / (alpha0 + alpha1 + alpha2);
     }
     if (w0_out) { *w0_out = omega0; *w2_out = omega2; }
-    return div_c<R>(omega0 * (a - R(2) * b + c), R(3), R(1.0 / 3.0)) +
-           div_c<R>((omega2 - R(0.5)) * (b - R(2) * c + d), R(6), R(1.0 / 6.0));
+    // a - 2b is a - RN(2b) with 2b exact, so one fused multiply-add gives
+    // its bits
+    return div_c<R>(omega0 * (fma(R(-2), b, a) + c), R(3), R(1.0 / 3.0)) +
+           div_c<R>((omega2 - R(0.5)) * (fma(R(-2), c, b) + d), R(6), R(1.0 / 6.0));
 #else
     // alpha_k = d_k / t_k with t_k = (eps+IS_k)^2: the normalised weights are
     // w0 = t1 t2 / S, w2 = 3 t0 t1 / S with S = t1 t2 + 6 t0 t2 + 3 t0 t1,
