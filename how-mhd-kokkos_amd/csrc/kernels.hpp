@@ -267,7 +267,10 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
     // of a zero, and a zero product added to dF leaves it unchanged (dF
     // starts at +0; a sum of doubles is -0 only when both addends are).  For
     // the same reason the structural zeros of R (columns 1 and 5 in rows 0,
-    // 1; column 3 in rows 4, 5) add nothing and are skipped.
+    // 1; column 3 in rows 4, 5) add nothing and are skipped.  The product
+    // R[r][m] * F_s is rounded once whenever it is formed, so forming the
+    // products of waves 4-6 early and adding them after wave 3 gives dF the
+    // same bits.
     auto add_col = [&](auto mc, const R* Rc, R Fs) {
         constexpr int M = decltype(mc)::value;
 #pragma unroll
@@ -277,7 +280,16 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
             dFo[r] += Rc[r] * Fs;
         }
     };
-    R Fs4 = R(0), Fs5 = R(0), Fs6 = R(0);
+    // products R[r][m] * F_s of waves 4, 5, 6, waiting for wave 3
+    R P4[7], P5[7], P6[7];
+    auto add_prod = [&](auto mc, const R* Pm) {
+        constexpr int M = decltype(mc)::value;
+#pragma unroll
+        for (int r = 0; r < 7; ++r) {
+            if (M == 5 && r < 2) continue;
+            dFo[r] += Pm[r];
+        }
+    };
     sfor<0, 4>([&](auto pc) {
         constexpr int P = decltype(pc)::value;
         constexpr int ma = P;                // 0, 1, 2, 3
@@ -353,16 +365,19 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
         R Rc[7];
         r_col<ma>(gam, e, Rc, gc);
         add_col(std::integral_constant<int, ma>{}, Rc, Fsa);
-        if constexpr (P == 0) Fs6 = Fsb;
-        else if constexpr (P == 1) Fs5 = Fsb;
-        else if constexpr (P == 2) Fs4 = Fsb;
-        else {
-            r_col<4>(gam, e, Rc, gc);
-            add_col(std::integral_constant<int, 4>{}, Rc, Fs4);
-            r_col<5>(gam, e, Rc, gc);
-            add_col(std::integral_constant<int, 5>{}, Rc, Fs5);
-            r_col<6>(gam, e, Rc, gc);
-            add_col(std::integral_constant<int, 6>{}, Rc, Fs6);
+        if constexpr (two) {
+            // column mb shares most of its products with column ma (built
+            // together, so the compiler forms them once); its products with
+            // the wave flux are added to dF after wave 3's
+            R Rb[7];
+            r_col<mb>(gam, e, Rb, gc);
+            R* Pb = (P == 0) ? P6 : (P == 1) ? P5 : P4;
+#pragma unroll
+            for (int r = 0; r < 7; ++r) Pb[r] = Rb[r] * Fsb;
+        } else {
+            add_prod(std::integral_constant<int, 4>{}, P4);
+            add_prod(std::integral_constant<int, 5>{}, P5);
+            add_prod(std::integral_constant<int, 6>{}, P6);
         }
     });
 }
