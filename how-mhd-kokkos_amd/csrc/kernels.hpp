@@ -60,6 +60,18 @@ __device__ __forceinline__ long long lin(const Geom& g, int x, int y, int z) {
     return ((long long)z * g.n[1] + y) * g.n[0] + x;
 }
 
+// Streaming stores and loads.  The arrays a kernel writes here (interface
+// fluxes, CT terms, EMFs, faces, q) are read again only by a later kernel,
+// after more than the L2 and the 256 MB Infinity Cache have passed through,
+// so they are written non-temporally and leave those caches to the stencil
+// re-reads; the stage combination reads q(0) and q(iw-1) once the same way.
+// Measured at 256^3 (FP64): face update 0.368 -> 0.345 ms, combination
+// 1.029 -> 1.004 ms, step 36.8 -> 36.7 ms.
+template <typename T>
+__device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+
 // Sweep permutation of src/ssprk.cpp:381-403, 1035-1065, 1126-1156: sweep slot m
 // holds q variable PERM[dir][m]; order (rho, Mn, Mt1, Mt2, Bn, Bt1, Bt2, E).
 __device__ __constant__ const int kPerm[3][8] = {
@@ -574,7 +586,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
 
 #pragma unroll
     for (int c = 0; c < 7; ++c)
-        if (!(A.ct_active && (c == 4 || c == 5))) A.dF[dF_index<DIR>(g, c, t, x, y, z)] = dF[c];
+        if (!(A.ct_active && (c == 4 || c == 5))) st_nt(&A.dF[dF_index<DIR>(g, c, t, x, y, z)], dF[c]);
 
     // CT flux terms, src/ssprk.cpp:967-994: (-0 a + 9 b + 9 c - 0 d)/18 over
     // cells k = 1..4.  The products with the zero weights are +-0 and are
@@ -595,8 +607,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
         if constexpr (DIR == 0) ci = ((long long)z * g.n[1] + y) * g.n[0] + (t - 1);
         else if constexpr (DIR == 1) ci = ((long long)z * g.n[1] + (t - 1)) * g.n[0] + x;
         else ci = ((long long)(t - 1) * g.n[1] + y) * g.n[0] + x;
-        A.ct_a[ci] = dF[4] + ftmp1;
-        A.ct_b[ci] = dF[5] + ftmp2;
+        st_nt(&A.ct_a[ci], dF[4] + ftmp1);
+        st_nt(&A.ct_b[ci], dF[5] + ftmp2);
     }
     if constexpr (MODE == SW_COLLECT) {
         if (!A.feat) return;
@@ -918,10 +930,10 @@ __global__ __launch_bounds__(EMF_THREADS) void k_emf_march(EmfMarchArgs<R> A) {
             const int c = z & 1;
             const long long i = lin(g, xm, ym, z);
             const R ox = sO[c][0][ey][ex], oy = yc, oz = zc;
-            A.Ox[i] = ox + lap24<R>(sO[c][0][ey][exl], ox, sO[c][0][ey][exr]) +
-                      lap24<R>(sO[c][0][eyl][ex], ox, sO[c][0][eyr][ex]);
-            A.Oy[i] = oy + lap24<R>(sO[c][1][eyl][ex], oy, sO[c][1][eyr][ex]) + lap24<R>(ymv, oy, yn);
-            A.Oz[i] = oz + lap24<R>(sO[c][2][ey][exl], oz, sO[c][2][ey][exr]) + lap24<R>(zmv, oz, zn);
+            st_nt(&A.Ox[i], ox + lap24<R>(sO[c][0][ey][exl], ox, sO[c][0][ey][exr]) +
+                                lap24<R>(sO[c][0][eyl][ex], ox, sO[c][0][eyr][ex]));
+            st_nt(&A.Oy[i], oy + lap24<R>(sO[c][1][eyl][ex], oy, sO[c][1][eyr][ex]) + lap24<R>(ymv, oy, yn));
+            st_nt(&A.Oz[i], oz + lap24<R>(sO[c][2][ey][exl], oz, sO[c][2][ey][exr]) + lap24<R>(zmv, oz, zn));
         }
         __syncthreads();
         ymv = yc; zmv = zc;
@@ -1041,9 +1053,9 @@ __global__ __launch_bounds__(256) void k_faces(FaceArgs<R> A) {
             bz = s.k1 * binit[2] + s.k3 * A.bp[2 * g.ncell + i] - s.dtdx * dOz + s.dtdy * dOy;
         }
     }
-    A.bo[i] = bx;
-    A.bo[g.ncell + i] = by;
-    A.bo[2 * g.ncell + i] = bz;
+    st_nt(&A.bo[i], bx);
+    st_nt(&A.bo[g.ncell + i], by);
+    st_nt(&A.bo[2 * g.ncell + i], bz);
 }
 
 // k_faces for three-dimensional grids (every CT flag set), marching a
@@ -1169,9 +1181,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOWMHD_FACE
             const double dyOy = d4((double)sOy[ty][tx], (double)sOy[ty + 1][tx], (double)sOy[ty + 2][tx],
                                    (double)sOy[ty + 3][tx], (double)sOy[ty + 4][tx], (double)sOy[ty + 5][tx]);
             const double bz = s.k1 * binit[2] + s.k3 * bp[2] - s.dtdx * row6(sOz[ty]) + s.dtdy * dyOy;
-            A.bo[i] = bx;
-            A.bo[g.ncell + i] = by;
-            A.bo[2 * g.ncell + i] = bz;
+            st_nt(&A.bo[i], bx);
+            st_nt(&A.bo[g.ncell + i], by);
+            st_nt(&A.bo[2 * g.ncell + i], bz);
         }
         __syncthreads();
 #pragma unroll
@@ -1287,7 +1299,7 @@ __device__ __forceinline__ void combine_cell(const CombineArgs<R>& A, long long 
     for (int m = 0; m < 8; ++m) {
         const long long o = (long long)m * g.ncell + i;
         v[m] = 0.0;
-        if (upd(m)) v[m] = s.stage5 ? A.q0[o] + s.c4 * A.q2[o] + s.c5 * A.q3[o] : A.q0[o];
+        if (upd(m)) v[m] = s.stage5 ? ld_nt(&A.q0[o]) + s.c4 * ld_nt(&A.q2[o]) + s.c5 * ld_nt(&A.q3[o]) : ld_nt(&A.q0[o]);
     }
     // slot of each q variable in the dF of each sweep (-1: the normal field)
     // x: (rho, Mx, My, Mz, Bx, By, Bz, E) -> (0,1,2,3,-,4,5,6)
@@ -1305,7 +1317,7 @@ __device__ __forceinline__ void combine_cell(const CombineArgs<R>& A, long long 
             double d = 0.0;
             if (sx[m] >= 0)
                 d = -dt * ((double)A.dFx[dF_index<0>(g, sx[m], tr, x, y, z)] - (double)A.dFx[dF_index<0>(g, sx[m], tl, x, y, z)]);
-            v[m] = s.k1 * v[m] + s.k3 * A.qp[(long long)m * g.ncell + i] + d;
+            v[m] = s.k1 * v[m] + s.k3 * ld_nt(&A.qp[(long long)m * g.ncell + i]) + d;
         }
     }
     if (ny > 1) {
@@ -1387,7 +1399,7 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<R> A) {
         double v[8];
         combine_cell<R>(A, i, x, y, z, v, rho_floor, p_floor);
 #pragma unroll
-        for (int m = 0; m < 8; ++m) A.qo[(long long)m * g.ncell + i] = v[m];
+        for (int m = 0; m < 8; ++m) st_nt(&A.qo[(long long)m * g.ncell + i], v[m]);
         if (A.cfl) cfl_cell(v, A.ph, m0, m1, m2);
     }
     if (A.cfl) {
