@@ -1128,9 +1128,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOWMHD_FACE
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long o = (long long)c * g.ncell + i;
-            fb0[c] = A.b0[o];
-            fbp[c] = A.bp[o];
-            if (s.stage5) { fb2[c] = A.b2[o]; fb3[c] = A.b3[o]; }
+            fb0[c] = ld_nt(&A.b0[o]);
+            fbp[c] = ld_nt(&A.bp[o]);
+            if (s.stage5) { fb2[c] = ld_nt(&A.b2[o]); fb3[c] = ld_nt(&A.b3[o]); }
             else { fb2[c] = 0.0; fb3[c] = 0.0; }
         }
     };
