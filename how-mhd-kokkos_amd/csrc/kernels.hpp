@@ -265,8 +265,9 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
 
     // The waves are projected in four passes over the stencil in LDS,
     // (0,6), (1,5), (2,4) and (3), and dF still gets them added in order 0..6
-    // (the fluxes of waves 4, 5, 6 wait for wave 3).  A pass shares work
-    // between its two rows of L (src/ssprk.cpp:662-716):
+    // (the products of waves 4, 5, 6 with their R columns wait for wave 3).
+    // A pass shares work between its two rows of L (src/ssprk.cpp:662-716)
+    // and its two columns of R (729-787):
     //  * rows 0 and 6, and rows 2 and 4, have equal entries 4..6, so those
     //    three products of a stencil cell serve both rows;
     //  * rows 1 and 5 have equal entries 0, 2, 3, opposite entries 4, 5 and
