@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace howmhd {
 
@@ -177,6 +178,14 @@ __device__ __forceinline__ void relu_pack(const f32x16& c, u32x4& lo, u32x4& hi)
     hi = u32x4{relu_bf2(c[8], c[9]), relu_bf2(c[10], c[11]), relu_bf2(c[12], c[13]), relu_bf2(c[14], c[15])};
 }
 
+template <int B, int E, typename F>
+__device__ __forceinline__ void sfor_mlp(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor_mlp<B + 1, E>(f);
+    }
+}
+
 __device__ __forceinline__ float mlp_forward_wave(const float* feat, const MlpDev& m) {
     const int l = threadIdx.x & 63;
     const int h = l >> 5;
@@ -198,8 +207,19 @@ __device__ __forceinline__ float mlp_forward_wave(const float* feat, const MlpDe
     const u32x4* w1 = reinterpret_cast<const u32x4*>(&m.W1f[l][0][0]);
     const u32x4* w2 = reinterpret_cast<const u32x4*>(&m.W2f[l][0][0]);
     const u32x4* w3 = reinterpret_cast<const u32x4*>(&m.W3f[l][0][0]);
-    const f32x16 b1 = *reinterpret_cast<const f32x16*>(&m.b1r[h][0]);
-    const f32x16 b2 = *reinterpret_cast<const f32x16*>(&m.b2r[h][0]);
+    // The biases in accumulator-row order differ only between the half-waves.
+    // Each lane loads one value per layer, row l % 16 of its half, and a DPP
+    // row_newbcast:r copies lane r of every 16-lane row to the whole row, so
+    // every lane of half h gets row r of half h: two 4-byte loads per lane
+    // instead of eight 16-byte ones.
+    const int vb1 = __float_as_int(m.b1r[h][l & 15]);
+    const int vb2 = __float_as_int(m.b2r[h][l & 15]);
+    f32x16 b1, b2;
+    sfor_mlp<0, 16>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        b1[r] = __int_as_float(__builtin_amdgcn_update_dpp(0, vb1, 0x150 + r, 0xf, 0xf, false));
+        b2[r] = __int_as_float(__builtin_amdgcn_update_dpp(0, vb2, 0x150 + r, 0xf, 0xf, false));
+    });
     const u32x4 w1a = w1[0], w1b = w1[1], w2a = w2[0], w2b = w2[1], w3a = w3[0], w3b = w3[1];
     float z[2];
 #pragma unroll
