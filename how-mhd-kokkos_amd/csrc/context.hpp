@@ -25,6 +25,10 @@ struct howmhd_gpu_ctx {
     howmhd::CtFlags ctf{};
     size_t ncell = 0;
     size_t rsize = 8;          // bytes of the sweep/CT real type
+    // tiles per block of the marching y/z sweeps when positive (environment
+    // variable HOWMHD_SWEEP_MARCH at create, for tests of the march on small
+    // grids); 0: chosen from the launch size
+    int march_force = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     double* qbuf[5] = {};      // 8 x ncell each

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -241,6 +242,7 @@ int howmhd_gpu_create(const howmhd_params* p, howmhd_gpu_ctx** out) {
     c->ctf.use_Oz = c->ctf.use_x && c->ctf.use_z;
     c->ctf.active = c->ctf.use_Ox || c->ctf.use_Oy || c->ctf.use_Oz;
     c->rsize = (p->precision == HOWMHD_PREC_FP32) ? sizeof(float) : sizeof(double);
+    if (const char* e = std::getenv("HOWMHD_SWEEP_MARCH")) c->march_force = std::max(0, std::atoi(e));
 
     auto cleanup_fail = [&](int code, const std::string& msg) {
         (void)hipGetLastError();  // reported here, not at a later launch check

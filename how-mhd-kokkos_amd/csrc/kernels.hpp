@@ -73,12 +73,15 @@ template <typename T>
 __device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
 
 // Sweep permutation of src/ssprk.cpp:381-403, 1035-1065, 1126-1156: sweep slot m
-// holds q variable PERM[dir][m]; order (rho, Mn, Mt1, Mt2, Bn, Bt1, Bt2, E).
-__device__ __constant__ const int kPerm[3][8] = {
-    {0, 1, 2, 3, 4, 5, 6, 7},
-    {0, 2, 3, 1, 5, 6, 4, 7},
-    {0, 3, 1, 2, 6, 4, 5, 7},
-};
+// holds q variable sweep_perm(dir, m); order (rho, Mn, Mt1, Mt2, Bn, Bt1, Bt2,
+// E): x {0,1,2,3,4,5,6,7}, y {0,2,3,1,5,6,4,7}, z {0,3,1,2,6,4,5,7}.  A
+// constant expression, so every use folds to an immediate offset.
+__host__ __device__ constexpr int sweep_perm(int dir, int m) {
+    return (m == 0 || m == 7) ? m : (m < 4 ? 1 + (m - 1 + dir) % 3 : 4 + (m - 4 + dir) % 3);
+}
+static_assert(sweep_perm(1, 1) == 2 && sweep_perm(1, 3) == 1 && sweep_perm(1, 6) == 4 &&
+              sweep_perm(2, 1) == 3 && sweep_perm(2, 2) == 1 && sweep_perm(2, 4) == 6 &&
+              sweep_perm(0, 5) == 5 && sweep_perm(2, 7) == 7, "sweep permutation");
 
 struct Phys {
     double gam, rhomin, pgmin;
@@ -164,6 +167,9 @@ struct SweepArgs {
     unsigned* fix;
     unsigned* fixlist;
     unsigned* fixctl;
+    // y/z sweeps: consecutive tiles along the sweep axis per block
+    // (sweep_march; 1: single tiles)
+    int march;
 };
 
 // dF layout per direction: component-major, interface index t = 0..n along the
@@ -404,97 +410,67 @@ __device__ __forceinline__ void weno_interface(const UF<R> (*luf)[SweepTile<DIR>
 template <typename R, int MODE>
 constexpr int sweep_waves() { return (sizeof(R) == 4 && MODE != SW_COLLECT) ? HOWMHD_ROUTED_WAVES : HOWMHD_SWEEP_WAVES; }
 
-// One tile (bx, by, bz) of the sweep grid (sweep_grid), block-linear index
-// blk.  DM (DivMode) of the WENO weights: the FMA forms and the
-// reference-rounding recompute launch divide with IEEE divisions; the
-// reference-rounding sweep proper with div_recip/div_with, queueing its tile
-// for the recompute launch where an operand leaves their range.
-template <typename R, int DIR, int MODE, int EX, int DM>
-__device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by, int bz, long long blk) {
+// y/z sweeps that march through consecutive tiles (sweep_march): the FP64
+// ones but the collecting sweep.  The FP32 sweep at 4 waves per SIMD and the
+// collecting sweep have no registers for the march's state and would spill.
+template <typename R, int MODE>
+constexpr bool sweep_marches() { return sizeof(R) == 8 && MODE != SW_COLLECT; }
+
+// Stored index of slot s of the LDS tile whose first interface is t0 (the
+// tile's cells run from t0 - 3 to t0 + ROWS + 1 along the sweep axis, x
+// sweep: t0 - 3 to t0 + LANES + 1), through the boundary map.
+template <int DIR>
+__device__ __forceinline__ long long tile_cell_index(const Geom& g, int s, int t0, int x0, int y0, int z0) {
     using T = SweepTile<DIR>;
-    __shared__ UF<R> luf[7][T::CELLS];
-    __shared__ R lo[O_COUNT][T::CELLS];
-    // |B| of each cell for the indicator features (routed and sample sweeps)
-    __shared__ float lbmag[MODE != SW_PLAIN ? T::CELLS : 1];
+    const int nd = g.n[DIR];
+    int cx, cy, cz, along;
+    if constexpr (DIR == 0) {
+        const int row = s / (T::LANES + 5), c = s % (T::LANES + 5);
+        along = t0 - 3 + c;
+        cy = y0 + row; cz = z0; cx = 0;
+        if (cy >= g.n[1]) cy = g.n[1] - 1;
+    } else {
+        const int row = s / T::LANES, lane = s % T::LANES;
+        along = t0 - 3 + row;
+        cx = x0 + lane;
+        if (cx >= g.n[0]) cx = g.n[0] - 1;
+        if constexpr (DIR == 1) { cz = z0; cy = 0; }
+        else { cy = y0; cz = 0; }
+    }
+    if (along > nd + 2) along = nd + 2;
+    const int a = bmap(along, nd, g.per[DIR]);
+    if constexpr (DIR == 0) cx = a;
+    else if constexpr (DIR == 1) cy = a;
+    else cz = a;
+    return ((long long)cz * g.n[1] + cy) * g.n[0] + cx;
+}
+
+// Cell values of one cell (its 8 conserved values in sweep order) into LDS
+// slot s.
+template <typename R, int DIR, int MODE, int DM>
+__device__ __forceinline__ void store_cell(const SweepArgs<R>& A, UF<R> (*luf)[SweepTile<DIR>::CELLS],
+                                           R (*lo)[SweepTile<DIR>::CELLS], float* lbmag, int s, const double* q,
+                                           bool& cell_bad) {
+    R out[C_NFIELD];
+    cell_values<R, DM>(q, A.ph.gam, A.ph.rhomin, A.ph.pgmin, out, &cell_bad);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
+#pragma unroll
+    for (int f = 0; f < O_COUNT; ++f) lo[f][s] = out[C_VN + f];
+    if constexpr (MODE != SW_PLAIN)
+        lbmag[s] = sqrtf((float)(out[C_BN] * out[C_BN] + out[C_BT1] * out[C_BT1] + out[C_BT2] * out[C_BT2]));
+}
+
+// Phase 2 of a tile whose cell values are in LDS: one interface per thread
+// (t0, x0, y0, z0: the tile origin; blk: its index in the single-tile grid).
+template <typename R, int DIR, int MODE, int EX, int DM>
+__device__ __forceinline__ void sweep_interfaces(const SweepArgs<R>& A, const UF<R> (*luf)[SweepTile<DIR>::CELLS],
+                                                 const R (*lo)[SweepTile<DIR>::CELLS], const float* lbmag, int t0,
+                                                 int x0, int y0, int z0, long long blk) {
+    using T = SweepTile<DIR>;
     const Geom& g = A.g;
     const int nd = g.n[DIR];
-    const int per = g.per[DIR];
-    const int tbeg = per ? 1 : 0;  // periodic: interface 0 is the image of n
     const int tid = threadIdx.x;
-
-    // block origin
-    int t0, x0 = 0, y0 = 0, z0 = 0;
-    if constexpr (DIR == 0) {
-        t0 = tbeg + bx * T::LANES;
-        y0 = by * T::ROWS;
-        z0 = bz;
-    } else {
-        x0 = bx * T::LANES;
-        t0 = tbeg + by * T::ROWS;
-        if constexpr (DIR == 1) z0 = bz;
-        else y0 = bz;
-    }
-
-    // ---- phase 1: cell values of the tile (stencil halo included) into LDS.
-    // A thread's cells are tid, tid + 256, ...; all their loads are issued
-    // before the first conversion, so a block waits on memory once.
-    auto cell_index = [&](int s) -> long long {
-        int cx, cy, cz, along;
-        if constexpr (DIR == 0) {
-            const int row = s / (T::LANES + 5), c = s % (T::LANES + 5);
-            along = t0 - 3 + c;
-            cy = y0 + row; cz = z0; cx = 0;
-            if (cy >= g.n[1]) cy = g.n[1] - 1;
-        } else {
-            const int row = s / T::LANES, lane = s % T::LANES;
-            along = t0 - 3 + row;
-            cx = x0 + lane;
-            if (cx >= g.n[0]) cx = g.n[0] - 1;
-            if constexpr (DIR == 1) { cz = z0; cy = 0; }
-            else { cy = y0; cz = 0; }
-        }
-        if (along > nd + 2) along = nd + 2;
-        const int a = bmap(along, nd, per);
-        if constexpr (DIR == 0) cx = a;
-        else if constexpr (DIR == 1) cy = a;
-        else cz = a;
-        return ((long long)cz * g.n[1] + cy) * g.n[0] + cx;
-    };
-    constexpr int NIT = (T::CELLS + 255) / 256;
-    double qs[NIT][8];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int s = tid + 256 * it;
-        if (s < T::CELLS) {
-            const long long idx = cell_index(s);
-#pragma unroll
-            for (int m = 0; m < 8; ++m) qs[it][m] = A.q[(long long)kPerm[DIR][m] * g.ncell + idx];
-        }
-    }
-    bool cell_bad = false;
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int s = tid + 256 * it;
-        if (s < T::CELLS) {
-            R out[C_NFIELD];
-            cell_values<R, DM>(qs[it], A.ph.gam, A.ph.rhomin, A.ph.pgmin, out, &cell_bad);
-#pragma unroll
-            for (int j = 0; j < 7; ++j) luf[j][s] = UF<R>{out[C_RHO + j], out[C_F0 + j]};
-#pragma unroll
-            for (int f = 0; f < O_COUNT; ++f) lo[f][s] = out[C_VN + f];
-            if constexpr (MODE != SW_PLAIN)
-                lbmag[s] = sqrtf((float)(out[C_BN] * out[C_BN] + out[C_BT1] * out[C_BT1] + out[C_BT2] * out[C_BT2]));
-        }
-    }
-    // a cell without positive shared reciprocals: the tile goes (once) on the
-    // list of the IEEE-division launch that follows
-    if constexpr (EX && DM == DIV_UNSCALED) {
-        if (__ballot(cell_bad) && atomicExch(&A.fix[blk], 1u) == 0u)
-            A.fixlist[atomicAdd(&A.fixctl[0], 1u)] = (unsigned)blk;
-    }
-    __syncthreads();
-
-    // ---- phase 2: one interface per thread
     int lane, row, t, x, y, z;
     bool valid;
     if constexpr (DIR == 0) {
@@ -622,6 +598,173 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by
     }
 }
 
+// One tile (bx, by, bz) of the sweep grid (sweep_grid), block-linear index
+// blk.  DM (DivMode) of the WENO weights: the FMA forms and the
+// reference-rounding recompute launch divide with IEEE divisions; the
+// reference-rounding sweep proper with div_recip/div_with, queueing its tile
+// for the recompute launch where an operand leaves their range.
+template <typename R, int DIR, int MODE, int EX, int DM>
+__device__ __forceinline__ void sweep_tile(const SweepArgs<R>& A, int bx, int by, int bz, long long blk) {
+    using T = SweepTile<DIR>;
+    __shared__ UF<R> luf[7][T::CELLS];
+    __shared__ R lo[O_COUNT][T::CELLS];
+    // |B| of each cell for the indicator features (routed and sample sweeps)
+    __shared__ float lbmag[MODE != SW_PLAIN ? T::CELLS : 1];
+    const Geom& g = A.g;
+    const int tbeg = g.per[DIR] ? 1 : 0;  // periodic: interface 0 is the image of n
+    const int tid = threadIdx.x;
+
+    // block origin
+    int t0, x0 = 0, y0 = 0, z0 = 0;
+    if constexpr (DIR == 0) {
+        t0 = tbeg + bx * T::LANES;
+        y0 = by * T::ROWS;
+        z0 = bz;
+    } else {
+        x0 = bx * T::LANES;
+        t0 = tbeg + by * T::ROWS;
+        if constexpr (DIR == 1) z0 = bz;
+        else y0 = bz;
+    }
+
+    // ---- phase 1: cell values of the tile (stencil halo included) into LDS.
+    // A thread's cells are tid, tid + 256, ...; all their loads are issued
+    // before the first conversion, so a block waits on memory once.
+    auto cell_index = [&](int s) -> long long { return tile_cell_index<DIR>(g, s, t0, x0, y0, z0); };
+    constexpr int NIT = (T::CELLS + 255) / 256;
+    double qs[NIT][8];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int s = tid + 256 * it;
+        if (s < T::CELLS) {
+            const long long idx = cell_index(s);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) qs[it][m] = A.q[(long long)sweep_perm(DIR, m) * g.ncell + idx];
+        }
+    }
+    bool cell_bad = false;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int s = tid + 256 * it;
+        if (s < T::CELLS) store_cell<R, DIR, MODE, DM>(A, luf, lo, lbmag, s, qs[it], cell_bad);
+    }
+    // a cell without positive shared reciprocals: the tile goes (once) on the
+    // list of the IEEE-division launch that follows
+    if constexpr (EX && DM == DIV_UNSCALED) {
+        if (__ballot(cell_bad) && atomicExch(&A.fix[blk], 1u) == 0u)
+            A.fixlist[atomicAdd(&A.fixctl[0], 1u)] = (unsigned)blk;
+    }
+    __syncthreads();
+
+    sweep_interfaces<R, DIR, MODE, EX, DM>(A, luf, lo, lbmag, t0, x0, y0, z0, blk);
+}
+
+// The y or z sweep of up to `nsteps` consecutive tiles along the sweep axis
+// (tiles by0, by0 + 1, ... of the single-tile grid) by one block.
+// Consecutive tiles share 5 of their 13 cell rows: each step after the first
+// moves rows 8..12 of the previous tile to rows 0..4 of the LDS tile and
+// converts only its 8 new rows, one cell per thread.  Every value is the one
+// sweep_tile computes for the same tile; queueing and statistics use the
+// tile's single-tile index.
+template <typename R, int DIR, int MODE, int EX, int DM>
+__device__ __forceinline__ void sweep_march(const SweepArgs<R>& A, int bx, int by0, int bz, int nsteps) {
+    static_assert(DIR != 0, "the x sweep takes single tiles");
+    using T = SweepTile<DIR>;
+    constexpr int HALO = 5 * T::LANES;        // slots of rows 0..4
+    constexpr int NEW = T::CELLS - HALO;      // slots of rows 5..12
+    constexpr int KEEP = T::ROWS * T::LANES;  // first slot of rows 8..12
+    static_assert(NEW == 256 && KEEP + HALO == T::CELLS, "one new cell per thread");
+    __shared__ UF<R> luf[7][T::CELLS];
+    __shared__ R lo[O_COUNT][T::CELLS];
+    __shared__ float lbmag[MODE != SW_PLAIN ? T::CELLS : 1];
+    const Geom& g = A.g;
+    const int tbeg = g.per[DIR] ? 1 : 0;
+    const int tid = threadIdx.x;
+    const dim3 tg = sweep_grid(g, DIR);  // the single-tile grid
+    const int x0 = bx * T::LANES;
+    int y0 = 0, z0 = 0;
+    if constexpr (DIR == 1) z0 = bz;
+    else y0 = bz;
+    static_assert(sweep_marches<R, MODE>(), "a sweep with registers for the march");
+    // Loads of the next step's cell issued before the interfaces of this
+    // step: only where the 16 registers they hold leave the interfaces
+    // without spills (the plain sweep of reference rounding); elsewhere each
+    // step loads its cell at its start.
+    constexpr bool EARLY = EX && MODE == SW_PLAIN;
+    double qn[8];            // this thread's new cell of the next step
+    bool bad_prev = false;   // this thread converted a cell out of range in the previous step
+    auto load_new = [&](int t0) {  // the new cell of the tile starting at row t0
+        const long long idx = tile_cell_index<DIR>(g, HALO + tid, t0, x0, y0, z0);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) qn[m] = A.q[(long long)sweep_perm(DIR, m) * g.ncell + idx];
+    };
+    // queueing of a tile (see sweep_tile) and the early loads
+    auto finish_cells = [&](bool cell_bad, int by, int t0, long long blk) {
+        if constexpr (EX && DM == DIV_UNSCALED) {
+            if (__ballot(cell_bad || bad_prev) && atomicExch(&A.fix[blk], 1u) == 0u)
+                A.fixlist[atomicAdd(&A.fixctl[0], 1u)] = (unsigned)blk;
+        }
+        bad_prev = cell_bad;
+        if constexpr (EARLY) {
+            if (by + 1 < by0 + nsteps && by + 1 < (int)tg.y) load_new(t0 + T::ROWS);
+        }
+    };
+    if (by0 >= (int)tg.y) return;  // uniform over the block
+    // first tile: every cell, as sweep_tile
+    {
+        const int t0 = tbeg + by0 * T::ROWS;
+        const long long blk = bx + (long long)tg.x * (by0 + (long long)tg.y * bz);
+        bool cell_bad = false;
+        constexpr int NIT = (T::CELLS + 255) / 256;
+        double qs[NIT][8];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int s = tid + 256 * it;
+            if (s < T::CELLS) {
+                const long long idx = tile_cell_index<DIR>(g, s, t0, x0, y0, z0);
+#pragma unroll
+                for (int m = 0; m < 8; ++m) qs[it][m] = A.q[(long long)sweep_perm(DIR, m) * g.ncell + idx];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int s = tid + 256 * it;
+            if (s < T::CELLS) store_cell<R, DIR, MODE, DM>(A, luf, lo, lbmag, s, qs[it], cell_bad);
+        }
+        finish_cells(cell_bad, by0, t0, blk);
+        __syncthreads();  // the tile's cells are in LDS
+        sweep_interfaces<R, DIR, MODE, EX, DM>(A, luf, lo, lbmag, t0, x0, y0, z0, blk);
+    }
+    for (int by = by0 + 1; by < by0 + nsteps && by < (int)tg.y; ++by) {  // uniform over the block
+        const int t0 = tbeg + by * T::ROWS;
+        const long long blk = bx + (long long)tg.x * (by + (long long)tg.y * bz);
+        if constexpr (!EARLY) load_new(t0);
+        __syncthreads();  // every interface of the previous tile has read it
+        // rows 8..12 of the previous tile are rows 0..4 of this one (source
+        // and destination slots are disjoint)
+        if (tid < HALO) {
+            const int src = KEEP + tid;
+            UF<R> u[7];
+            R o[O_COUNT];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) u[j] = luf[j][src];
+#pragma unroll
+            for (int f = 0; f < O_COUNT; ++f) o[f] = lo[f][src];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) luf[j][tid] = u[j];
+#pragma unroll
+            for (int f = 0; f < O_COUNT; ++f) lo[f][tid] = o[f];
+            if constexpr (MODE != SW_PLAIN) lbmag[tid] = lbmag[src];
+        }
+        __syncthreads();  // rows 8..12 are read before the new rows overwrite them
+        bool cell_bad = false;
+        store_cell<R, DIR, MODE, DM>(A, luf, lo, lbmag, HALO + tid, qn, cell_bad);
+        finish_cells(cell_bad, by, t0, blk);
+        __syncthreads();  // the tile's cells are in LDS
+        sweep_interfaces<R, DIR, MODE, EX, DM>(A, luf, lo, lbmag, t0, x0, y0, z0, blk);
+    }
+}
+
 // ---------------------------------------------------------------- CT kernels
 struct CtFlags {
     int use_x, use_y, use_z, use_Ox, use_Oy, use_Oz, active;
@@ -668,9 +811,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves
         }
     } else {
         int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
-        if constexpr (DIR == 0) xcd_row_tiles(bx, by, bz);
-        const long long blk = bx + (long long)gridDim.x * (by + (long long)gridDim.y * bz);
-        sweep_tile<R, DIR, MODE, EX, DM>(A, bx, by, bz, blk);
+        if constexpr (DIR != 0 && sweep_marches<R, MODE>()) {
+            sweep_march<R, DIR, MODE, EX, DM>(A, bx, by * A.march, bz, A.march);
+        } else {
+            if constexpr (DIR == 0) xcd_row_tiles(bx, by, bz);
+            const long long blk = bx + (long long)gridDim.x * (by + (long long)gridDim.y * bz);
+            sweep_tile<R, DIR, MODE, EX, DM>(A, bx, by, bz, blk);
+        }
     }
 }
 

@@ -5,6 +5,8 @@
 
 #include "context.hpp"
 
+#include <algorithm>
+
 namespace howmhd {
 // internal linkage: each translation unit keeps its own copy (the units
 // differ in HOWMHD_EXACT and contraction)
@@ -42,12 +44,32 @@ void launch_sweep_dm(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a, dim3 grid)
 // Blocks of the IEEE-division recompute launch of the reference-rounding form.
 constexpr unsigned kRecomputeBlocks = 64;
 
+// Tiles a y/z-sweep block marches through (sweep_march): as many as keep at
+// least kMarchMinBlocks blocks in the launch, at most HOWMHD_MARCH_STEPS.
+#ifndef HOWMHD_MARCH_STEPS
+#define HOWMHD_MARCH_STEPS 8
+#endif
+constexpr unsigned kMarchMinBlocks = 2048;
+inline int march_steps(const dim3& tiles) {
+    const unsigned long long n = (unsigned long long)tiles.x * tiles.y * tiles.z;
+    unsigned long long s = n / kMarchMinBlocks;
+    if (s > HOWMHD_MARCH_STEPS) s = HOWMHD_MARCH_STEPS;
+    if (s > tiles.y) s = tiles.y;
+    return s < 1 ? 1 : (int)s;
+}
+
 // The sweep along d.  Reference rounding: the sweep with unscaled WENO
 // divisions, then the IEEE-division launch for the tiles it queued (it exits
 // at once when there are none; no counting there).
 template <typename R, int MODE>
-void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a) {
-    const dim3 grid = sweep_grid(c->g, d);
+void launch_sweep(howmhd_gpu_ctx* c, int d, const SweepArgs<R>& a0) {
+    SweepArgs<R> a = a0;
+    dim3 grid = sweep_grid(c->g, d);
+    a.march = 1;
+    if (d != 0 && sweep_marches<R, MODE>()) {
+        a.march = c->march_force > 0 ? (int)std::min<unsigned>(c->march_force, grid.y) : march_steps(grid);
+        grid.y = (grid.y + a.march - 1) / a.march;
+    }
     if constexpr (EX) {
         launch_sweep_dm<R, MODE, DIV_UNSCALED>(c, d, a, grid);
         SweepArgs<R> b = a;

@@ -396,3 +396,29 @@ def test_fp32_indicator_full_size_invariants(require_gpu):
         assert st["routed_steps"] == 3 and 0.0 < st["flagged_fraction"] < 0.05
     finally:
         g.close()
+
+
+def test_routed_fp64_marching_sweeps_change_no_bit(monkeypatch, require_gpu):
+    """Mixed routing in the plain FP64 form with marching y/z sweeps (3 tiles
+    per block) equals single tiles bit for bit, flags and counts included."""
+    run, q0, p = _setup("mpi_smooth3d", nx=16, ny=40, nz=33, cour=0.8)
+    P = _params(5)
+    P[M.OFF_B3] = 0.0
+    print("[routed fp64 march] 16 x 40 x 33", flush=True)
+    solvers = []
+    for m in ("3", "1"):
+        monkeypatch.setenv("HOWMHD_SWEEP_MARCH", m)
+        g = H.GpuSolver(p)
+        g.set_indicator(P, 0.5, enable=2)
+        g.upload_q0(q0)
+        g.init_faces()
+        solvers.append(g)
+    for _ in range(2):
+        dt = solvers[1].tstep()
+        for g in solvers:
+            g.ssprk(dt)
+    a, b = (H.active(g.q0()) for g in solvers)
+    assert np.count_nonzero(a.view(np.int64) != b.view(np.int64)) == 0
+    sa, sb = (g.indicator_stats() for g in solvers)
+    print("stats", sa)
+    assert sa == sb and 0 < sa["flagged"] < sa["total"]

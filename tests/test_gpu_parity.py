@@ -427,3 +427,53 @@ def test_fp64_bitwise_random_states(seed, require_gpu):
     assert np.all(np.isfinite(H.active(qr)))
     assert _bitwise_mismatch(gpu.q0(), qr) == 0
     assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
+
+
+@pytest.mark.parametrize("march", [2, 3, 8])
+@pytest.mark.parametrize("bc_open", [False, True])
+def test_fp64_bitwise_marching_sweeps(march, bc_open, monkeypatch, require_gpu):
+    """The y and z sweeps with blocks marching through `march` consecutive
+    tiles (HOWMHD_SWEEP_MARCH; large grids choose it from the launch size):
+    random states on 12 x 40 x 33 (5 or 6 tiles along y and z, so the last
+    block of a march of 2 or 3 holds fewer tiles and a march of 8 is cut to
+    the tiles there are), two steps, bit for bit against the restatement."""
+    shape = (12, 40, 33)
+    bcs = dict(x1bc="open", x2bc="open", x3bc="open") if bc_open else {}
+    run, _, p = _setup("uniform", dict(nx=shape[0], ny=shape[1], nz=shape[2], cour=0.4,
+                                      gam="1.6666666666666667", **bcs))
+    q0 = _random_state(shape, 5 if bc_open else 4, bc_open)
+    print(f"[parity fp64 march] march={march} shape={shape} open={bc_open}", flush=True)
+    monkeypatch.setenv("HOWMHD_SWEEP_MARCH", str(march))
+    ref = O.RefSolver(p)
+    gpu = H.GpuSolver(p)
+    for s in (ref, gpu):
+        s.upload_q0(q0)
+        s.init_faces()
+    for step in range(2):
+        dt = ref.tstep()
+        assert gpu.tstep() == dt, f"dt differs at step {step}"
+        ref.ssprk(dt)
+        gpu.ssprk(dt)
+    assert _bitwise_mismatch(gpu.q0(), ref.q0()) == 0
+    assert _bitwise_mismatch(gpu.faces(), ref.faces()) == 0
+
+
+def test_fp64fast_marching_sweeps_change_no_bit(monkeypatch, require_gpu):
+    """fp64fast with marching y/z sweeps (3 tiles per block) equals the same
+    form with single tiles bit for bit: the march only moves cells within LDS."""
+    run, q0, p = _setup("mhd_blast3d", dict(nx=16, ny=40, nz=33, x3bc="open", cour=0.4))
+    p.precision = H.PREC["fp64fast"]
+    print("[fp64fast march] 16 x 40 x 33", flush=True)
+    solvers = []
+    for m in ("3", "1"):
+        monkeypatch.setenv("HOWMHD_SWEEP_MARCH", m)
+        solvers.append(H.GpuSolver(p))
+    for s in solvers:
+        s.upload_q0(q0)
+        s.init_faces()
+    for _ in range(2):
+        dt = solvers[1].tstep()
+        for s in solvers:
+            s.ssprk(dt)
+    assert _bitwise_mismatch(solvers[0].q0(), solvers[1].q0()) == 0
+    assert _bitwise_mismatch(solvers[0].faces(), solvers[1].faces()) == 0
