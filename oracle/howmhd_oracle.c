@@ -27,9 +27,13 @@
  * reference ships no golden data; tests/golden holds regression fixtures of
  * this file.
  *
+ * The -0.0 weights of the CT flux terms are kept as the reference writes
+ * them; tests/test_sweep_rewrites.py shows that leaving their products out
+ * changes no bit of the CT terms, so their presence is pinned by invariance.
+ *
  * Parity unpinned (only the reference text fixes them; this file reproduces
- * them by reading it): the operation order inside each formula, the -0.0
- * weights of the CT flux terms, the stage-5 k4/k1 and k5/k1 divisions, prot
+ * them by reading it): the operation order inside each formula, the
+ * stage-5 k4/k1 and k5/k1 divisions, prot
  * always rebuilding E, floors written back to the workspace only, the padded
  * wrap of bc_index_ct (it only touches ghost results that are refilled), and
  * which face or CT value the open fill copies into the ghost layers where the
